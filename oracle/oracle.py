@@ -1,0 +1,183 @@
+"""ctypes front end of the CPU FP64 oracle (liblowdin_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker or the timed CPU baseline.  The
+product package never imports this module.
+
+Each function follows the reference routine named in its docstring; the C
+side (lowdin_oracle.c) cites the reference lines.  Matrices come back as
+numpy arrays in the reference's row-major Eigen convention (element (r, c)
+at r*n + c), which for the symmetric results equals the Fortran image.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+DBL_EPSILON = float(np.finfo(np.float64).eps)
+
+
+def build():
+    """Compile the oracle (and oracle/_ref when the reference tree exists)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liblowdin_oracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        lp = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+        L.orc_boys.argtypes = [C.c_int, C.c_double, dp]
+        L.orc_prim_norm.argtypes = [C.c_double, C.c_int]
+        L.orc_prim_norm.restype = C.c_double
+        L.orc_prim_eri.argtypes = [C.c_int] * 4 + [dp] * 4 + [C.c_double] * 4 + [dp]
+        L.orc_eri_shell.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_double, dp]
+        L.orc_overlap_shell.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, dp]
+        L.orc_norma.argtypes = [C.c_void_p, dp]
+        L.orc_nbasis.argtypes = [C.c_void_p]
+        L.orc_nbasis.restype = C.c_int
+        L.orc_shellpair_list.argtypes = [C.c_void_p, C.c_double, ip, ip, dp]
+        L.orc_shellpair_list.restype = C.c_int
+        L.orc_schwarz.argtypes = [C.c_void_p, dp]
+        L.orc_compute_2body_direct.argtypes = [C.c_void_p, ip, ip, C.c_int, dp, dp, C.c_double,
+                                               C.c_int, C.c_double, C.c_double, C.c_int, dp, lp]
+        L.orc_compute_2body_direct.restype = C.c_long
+        L.orc_compute_coupling_direct.argtypes = [C.c_void_p, C.c_void_p, dp, C.c_int, C.c_double,
+                                                  C.c_int, dp, lp]
+        L.orc_1body.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, dp, dp]
+        _LIB = L
+    return _LIB
+
+
+class _CBasis(C.Structure):
+    _fields_ = [("nshell", C.c_int), ("l", C.c_void_p), ("nprim", C.c_void_p), ("poff", C.c_void_p),
+                ("exps", C.c_void_p), ("coefs", C.c_void_p), ("xyz", C.c_void_p)]
+
+
+class Basis:
+    """Shell arrays of one species, kept alive for the C side."""
+
+    def __init__(self, species):
+        self.species = species
+        self.l, self.nprim, self.poff, self.exps, self.coefs, self.xyz = species.arrays()
+        self.c = _CBasis(len(self.l), self.l.ctypes.data, self.nprim.ctypes.data, self.poff.ctypes.data,
+                         self.exps.ctypes.data, self.coefs.ctypes.data, self.xyz.ctypes.data)
+        self.ptr = C.cast(C.pointer(self.c), C.c_void_p)
+        self.nshell = len(self.l)
+        self.n = int(lib().orc_nbasis(self.ptr))
+        self.sizes = (self.l + 1) * (self.l + 2) // 2
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)[:-1]]).astype(int)
+
+
+def boys(mmax: int, T: float) -> np.ndarray:
+    F = np.zeros(mmax + 1)
+    lib().orc_boys(mmax, float(T), F)
+    return F
+
+
+def prim_norm(alpha: float, l: int) -> float:
+    return lib().orc_prim_norm(alpha, l)
+
+
+def prim_eri(la, lb, lc, ld, A, B, Cc, D, a, b, c, d) -> np.ndarray:
+    nc = lambda l: (l + 1) * (l + 2) // 2
+    out = np.zeros(nc(la) * nc(lb) * nc(lc) * nc(ld))
+    f = lambda x: np.ascontiguousarray(x, dtype=np.float64)
+    lib().orc_prim_eri(la, lb, lc, ld, f(A), f(B), f(Cc), f(D), a, b, c, d, out)
+    return out
+
+
+def eri_shell(b12: Basis, s1, s2, b34: Basis, s3, s4, prim_eps=0.0) -> np.ndarray:
+    """Raw contracted ERIs of (s1 s2|s3 s4), libint order (f1 slowest)."""
+    n = int(b12.sizes[s1] * b12.sizes[s2] * b34.sizes[s3] * b34.sizes[s4])
+    out = np.zeros(n)
+    lib().orc_eri_shell(b12.ptr, s1, s2, b34.ptr, s3, s4, prim_eps, out)
+    return out
+
+
+def norma(b: Basis) -> np.ndarray:
+    """Libint2Iface.cpp:115-127: 1/sqrt of the raw self-overlap diagonal."""
+    out = np.zeros(b.n)
+    lib().orc_norma(b.ptr, out)
+    return out
+
+
+def shellpair_list(b: Basis, threshold: float = 1e-12):
+    """compute_shellpair_list (Libint2Iface.cpp:3248-3342).  Returns
+    (pairs array [npair, 2] of (s1, s2 <= s1), norms [nshell, nshell])."""
+    ns = b.nshell
+    s1 = np.zeros(ns * (ns + 1) // 2, dtype=np.int32)
+    s2 = np.zeros_like(s1)
+    norms = np.zeros(ns * ns)
+    np_ = lib().orc_shellpair_list(b.ptr, threshold, s1, s2, norms)
+    return np.stack([s1[:np_], s2[:np_]], axis=1), norms.reshape(ns, ns)
+
+
+def schwarz(b: Basis) -> np.ndarray:
+    """compute_schwartz_ints (Libint2Iface.cpp:3345-3412), raw convention."""
+    Q = np.zeros(b.nshell * b.nshell)
+    lib().orc_schwarz(b.ptr, Q)
+    return Q.reshape(b.nshell, b.nshell)
+
+
+def compute_2body_direct(b: Basis, D: np.ndarray, factor: float, pairs=None, Q=None,
+                         precision: float = DBL_EPSILON, prim_eps: float = 0.0,
+                         nthreads: int = 0, return_counts: bool = False):
+    """LibintInterface::compute_2body_direct (Libint2Iface.cpp:418-618) with
+    the C wrapper's per-call pair list and Schwarz matrix (3505, 3520)."""
+    if pairs is None:
+        pairs, _ = shellpair_list(b)
+    if Q is None:
+        Q = schwarz(b)
+    s1 = np.ascontiguousarray(pairs[:, 0], dtype=np.int32)
+    s2 = np.ascontiguousarray(pairs[:, 1], dtype=np.int32)
+    G = np.zeros(b.n * b.n)
+    counters = np.zeros(2, dtype=np.int64)
+    screen = 1 if b.species.is_electron else 0
+    nints = lib().orc_compute_2body_direct(b.ptr, s1, s2, len(s1), np.ascontiguousarray(Q, dtype=np.float64).reshape(-1),
+                                           np.ascontiguousarray(D, dtype=np.float64).reshape(-1), float(factor),
+                                           screen, precision, prim_eps, nthreads, G, counters)
+    G = G.reshape(b.n, b.n)
+    if return_counts:
+        return G, {"eris": int(nints), "quartets": int(counters[0]), "prim_quartets": int(counters[1])}
+    return G
+
+
+def compute_coupling_direct(this: Basis, other: Basis, id_this: int, id_other: int,
+                            D_other: np.ndarray, drop: float = 1e-10, nthreads: int = 0,
+                            return_counts: bool = False):
+    """LibintInterface_compute_coupling_direct (Libint2Iface.cpp:3566-3589):
+    the lower species id runs the outer loop; ``permuted = id_this >
+    id_other``.  Returns the n_this x n_this image (lower triangle filled in
+    row-major = upper triangle of the Fortran column-major result)."""
+    permuted = id_this > id_other
+    outer, inner = (other, this) if permuted else (this, other)
+    n = this.n
+    B = np.zeros(n * n)
+    counts = np.zeros(2, dtype=np.int64)
+    lib().orc_compute_coupling_direct(outer.ptr, inner.ptr, np.ascontiguousarray(D_other, dtype=np.float64).reshape(-1),
+                                      1 if permuted else 0, drop, nthreads, B, counts)
+    B = B.reshape(n, n)
+    if return_counts:
+        return B, {"kept": int(counts[0]), "near_cutoff": int(counts[1])}
+    return B
+
+
+def one_body(b: Basis, kind: int, charges=()) -> np.ndarray:
+    """S (1), T (2) or point-charge attraction (3), normalised with norma."""
+    Z = np.array([z for z, _ in charges] or [0.0], dtype=np.float64)
+    X = np.array([x for _, x in charges] or [[0.0, 0.0, 0.0]], dtype=np.float64).reshape(-1)
+    M = np.zeros(b.n * b.n)
+    lib().orc_1body(b.ptr, kind, len(charges), Z, X, M)
+    return M.reshape(b.n, b.n)
