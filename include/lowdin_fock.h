@@ -1,0 +1,112 @@
+/*
+ * lowdin_fock.h -- C ABI of the MI355X direct two-particle Fock build.
+ *
+ * liblowdin_fock.so is a drop-in for the native half of openLOWDIN's direct
+ * SCF path.  The first block of entry points keeps the exact symbol names,
+ * argument order and meaning of the `extern "C"` functions that openLOWDIN's
+ * Fortran binds through ISO_C_BINDING (src/ints/Libint2Interface.f90:165-373),
+ * declared in src/ints/Libint2Iface.h:350-453 and defined in
+ * src/ints/Libint2Iface.cpp:3454-3589.  Behind them the integrals, the J/K
+ * contraction and the coupling contraction run as HIP kernels on the GPU.
+ *
+ * Conventions (as in the reference):
+ *   - one handle per quantum species; shells are added in particle order,
+ *     then basis-file contraction order, with RAW (file) coefficients;
+ *   - Cartesian functions only (xx, xy, xz, yy, yz, zz), each normalised to 1;
+ *   - all n x n matrices are dense FP64, Fortran column-major; D is assumed
+ *     symmetric; the callee overwrites every element of `result`;
+ *   - the entry points return nothing.  A failure leaves `result` filled
+ *     with NaN and is reported by lowdin_fock_last_error().
+ */
+#ifndef LOWDIN_FOCK_H
+#define LOWDIN_FOCK_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------ reference entry points */
+
+/* Libint2Iface.cpp:3454 (LibintInterface_new).  Creates the handle of one
+ * species.  stack_size is accepted and unused (it sized the disk path's
+ * integral stack); `parallel` is accepted and unused (the GPU replaces the
+ * OpenMP thread pool). */
+void* LibintInterface_new(int stack_size, int species_id, bool is_electron, bool parallel);
+
+/* Libint2Iface.cpp:3460 (LibintInterface_del). */
+void LibintInterface_del(void* lint);
+
+/* Libint2Iface.cpp:3467 (LibintInterface_add_particle).  z = -(total charge)
+ * of the particle; recorded, not used by the two-particle path. */
+void LibintInterface_add_particle(void* lint, int z, const double* center);
+
+/* Libint2Iface.cpp:3473 (LibintInterface_add_shell).  Exponents, raw
+ * contraction coefficients, origin (bohr), angular momentum l <= 2 and the
+ * number of primitives.  Computes norma = 1/sqrt(S_raw,ii) per Cartesian
+ * function (Libint2Iface.cpp:79-128). */
+void LibintInterface_add_shell(void* lint, double* alpha, double* coeff, double* origin, int l, int nprim);
+
+/* Declared at Libint2Iface.h:368 but never defined by the reference; defined
+ * here: the number of Cartesian basis functions of the species. */
+int LibintInterface_get_nbasis(void* lint);
+
+/* Libint2Iface.cpp:3505 (LibintInterface_init_2body_ints).  Builds the
+ * significant shell-pair list (same centre, or raw overlap Frobenius norm
+ * >= 1e-12; Libint2Iface.cpp:3248-3342) on the GPU. */
+void LibintInterface_init_2body_ints(void* lint);
+
+/* Libint2Iface.cpp:3510-3526 (LibintInterface_compute_2body_direct).
+ * result = J[D] + factor K[D] (n x n), with the reference's screening:
+ * unique shell quartets from the pair list, Dnorm * Q_ab * Q_cd < DBL_EPSILON
+ * skipped for electrons only, G = 0.25 (G + G^T).  `factor` is passed by
+ * address, as the Fortran binding does (real(c_double) without VALUE). */
+void LibintInterface_compute_2body_direct(void* lint, double* dens, double* result, double* factor);
+
+/* Libint2Iface.cpp:3566-3589 (LibintInterface_compute_coupling_direct).
+ * result (n_lint x n_lint) = Coulomb matrix on lint's species from the
+ * density `dens` (n_olint x n_olint) of olint's species, every shell pair of
+ * both species (no screening), raw integrals below 1e-10 dropped, filled in
+ * the upper triangle of the column-major result (the caller symmetrises). */
+void LibintInterface_compute_coupling_direct(void* lint, void* olint, double* dens, double* result);
+
+/* ------------------------------------------------ extensions (no reference
+ * counterpart; used by this package's host code, tests and benchmarks) */
+
+/* Message of the last failure in this thread ("" if none). */
+const char* lowdin_fock_last_error(void);
+
+typedef struct {
+  uint64_t quartets;        /* shell quartets evaluated by the last build */
+  uint64_t eris;            /* contracted ERIs (the reference's num_ints_computed) */
+  uint64_t prim_quartets;   /* primitive quartets evaluated */
+  double kernel_ms;         /* device time of the integral kernels (HIP events) */
+  double total_ms;          /* wall time of the call */
+} lowdin_fock_stats;
+
+/* Counters of the last compute_2body_direct / compute_coupling_direct call
+ * on this handle.  Returns 0 on success. */
+int lowdin_fock_get_stats(void* lint, lowdin_fock_stats* out);
+
+/* The shell-pair list built by init_2body_ints as (s1, s2 <= s1) pairs,
+ * s1 ascending then s2 ascending.  Writes at most cap pairs; returns the
+ * total count (or -1). */
+int lowdin_fock_shellpair_list(void* lint, int* s1, int* s2, int cap);
+
+/* The raw Schwarz matrix Q (nshell x nshell, row-major; zero for pairs not
+ * in the pair list).  Returns 0 on success. */
+int lowdin_fock_schwarz(void* lint, double* Q);
+
+/* Both directions of one species pair from a single pass over the quartets:
+ * ra (n_a^2) = coupling on a from db, rb (n_b^2) = coupling on b from da,
+ * each in the layout of LibintInterface_compute_coupling_direct.  Either
+ * result may be NULL.  Returns 0 on success. */
+int lowdin_fock_coupling_pair(void* a, void* b, const double* da, const double* db, double* ra, double* rb);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LOWDIN_FOCK_H */

@@ -1,0 +1,416 @@
+// Templates of the HIP kernels of the direct two-particle Fock build and the inter-species
+// coupling build, written for gfx950 (CDNA4, 64-wide wavefronts).
+//
+// Work decomposition: a wavefront owns one bra shell pair and up to 64 ket
+// shell pairs of one quartet class (one ket pair per lane).  All bra data
+// (pair record, primitive pairs) are wave-uniform and are fetched through
+// the scalar unit; the per-lane ket data are vector loads.  Each lane
+// evaluates its contracted quartet with Obara-Saika VRR over its primitive
+// quartets, contracts to [e0|f0] in registers, runs the HGP HRR once, and
+// folds the integrals straight into the J/K (or coupling) blocks.  The J
+// block of the shared bra pair is reduced across the wavefront before one
+// atomic per element; the other five blocks differ per lane and go to HBM
+// with hardware FP64 atomics.
+//
+// Reference behaviour reproduced (openLOWDIN src/ints/Libint2Iface.cpp):
+//   quartet screen Dnorm * Q_ab * Q_cd < eps for electrons only (523-534),
+//   permutational degeneracy (543-546), the six J/K updates with the
+//   0.5*factor exchange weight (562-586), G = 0.25 (G + G^T) (616);
+//   coupling: |raw| < 1e-10 drop, f1 <= f2 / of1 <= of2 element sets, the
+//   doubling of off-diagonal density elements, lower-triangle (row-major)
+//   result (1179-1266).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/amd_detail/amd_hip_unsafe_atomics.h>
+
+#include "lowdin_device.h"
+#include "lowdin_kernels.h"
+
+namespace lowdin {
+
+__host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+
+// -------------------------------------------------------------- Boys F_m(T)
+// Taylor expansion of F_L around the nearest tabulated point, exp(-T) the
+// same way, then downward recursion F_m = (2T F_{m+1} + e^{-T}) / (2m+1).
+template <int L>
+__device__ __forceinline__ void boys_f64(double T, const double* __restrict__ tab, double* F) {
+  if (T < kBoysTmax) {
+    const int i = (int)(T * kBoysInvH + 0.5);
+    const double x = (double)i * kBoysH - T;
+    const double* r = tab + i * kBoysCols;
+    double s = r[1 + L + 7];
+    s = fma(s, x * (1.0 / 7.0), r[1 + L + 6]);
+    s = fma(s, x * (1.0 / 6.0), r[1 + L + 5]);
+    s = fma(s, x * (1.0 / 5.0), r[1 + L + 4]);
+    s = fma(s, x * (1.0 / 4.0), r[1 + L + 3]);
+    s = fma(s, x * (1.0 / 3.0), r[1 + L + 2]);
+    s = fma(s, x * (1.0 / 2.0), r[1 + L + 1]);
+    s = fma(s, x, r[1 + L + 0]);
+    F[L] = s;
+    if (L > 0) {
+      double e = 1.0;
+      e = fma(e, x * (1.0 / 7.0), 1.0);
+      e = fma(e, x * (1.0 / 6.0), 1.0);
+      e = fma(e, x * (1.0 / 5.0), 1.0);
+      e = fma(e, x * (1.0 / 4.0), 1.0);
+      e = fma(e, x * (1.0 / 3.0), 1.0);
+      e = fma(e, x * (1.0 / 2.0), 1.0);
+      e = fma(e, x, 1.0);
+      const double eT = r[0] * e;
+      const double t2 = 2.0 * T;
+#pragma unroll
+      for (int m = L - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + eT) * (1.0 / (2 * m + 1));
+    }
+  } else {
+    // erf(sqrt(T)) = 1 to 4e-19 here; the upward recursion keeps its
+    // exp(-T) term, which is not negligible against F_m for m ~ T/5.
+    F[0] = 0.88622692545275801365 * rsqrt(T);
+    if (L > 0) {
+      const double o2t = 0.5 / T;
+      const double eT = exp(-T);
+#pragma unroll
+      for (int m = 0; m < L; ++m) F[m + 1] = ((double)(2 * m + 1) * F[m] - eT) * o2t;
+    }
+  }
+}
+
+// ---------------------------------------------------------- class dispatch
+template <int LA, int LB, int LC, int LD>
+struct EriClass;
+#define LOWDIN_ERI_CLASS(a, b, c, d)                                                                 \
+  template <>                                                                                        \
+  struct EriClass<a, b, c, d> {                                                                      \
+    static constexpr int nacc = NACC_##a##b##c##d;                                                   \
+    template <typename R>                                                                            \
+    __device__ static __forceinline__ void vrr(const R* F, R PAx, R PAy, R PAz, R WPx, R WPy, R WPz, \
+                                               R QCx, R QCy, R QCz, R WQx, R WQy, R WQz, R o2p,      \
+                                               R o2q, R o2pq, R rop, R roq, R* acc) {                \
+      vrr_##a##b##c##d<R>(F, PAx, PAy, PAz, WPx, WPy, WPz, QCx, QCy, QCz, WQx, WQy, WQz, o2p, o2q,   \
+                          o2pq, rop, roq, acc);                                                      \
+    }                                                                                                \
+    template <typename R>                                                                            \
+    __device__ static __forceinline__ void hrr(const R* acc, R ABx, R ABy, R ABz, R CDx, R CDy,      \
+                                               R CDz, R* out) {                                      \
+      hrr_##a##b##c##d<R>(acc, ABx, ABy, ABz, CDx, CDy, CDz, out);                                   \
+    }                                                                                                \
+  };
+
+
+// Contracted raw quartet (canonical orientation) for one lane: loops over
+// the bra's (uniform) and the ket's primitive pairs, VRR into acc, HRR.
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void contracted_quartet(const double* __restrict__ bprim, int nbp,
+                                                   const double* __restrict__ kprim, int nkp,
+                                                   const double* __restrict__ boys,
+                                                   double ABx, double ABy, double ABz, double CDx,
+                                                   double CDy, double CDz, double* out) {
+  using C = EriClass<LA, LB, LC, LD>;
+  constexpr int L = LA + LB + LC + LD;
+  double acc[C::nacc];
+#pragma unroll
+  for (int k = 0; k < C::nacc; ++k) acc[k] = 0.0;
+  for (int ip = 0; ip < nbp; ++ip) {
+    const double* bp = bprim + ip * kPrimPairStride;
+    const double p = bp[0], o2p = bp[1];
+    const double Px = bp[2], Py = bp[3], Pz = bp[4];
+    const double PAx = bp[5], PAy = bp[6], PAz = bp[7];
+    const double Kab = bp[8];
+    for (int kp = 0; kp < nkp; ++kp) {
+      const double* kq = kprim + kp * kPrimPairStride;
+      const double q = kq[0], o2q = kq[1];
+      const double PQx = Px - kq[2], PQy = Py - kq[3], PQz = Pz - kq[4];
+      const double QCx = kq[5], QCy = kq[6], QCz = kq[7];
+      const double inv = 1.0 / (p + q);
+      const double rop = q * inv, roq = p * inv;
+      const double rho = p * rop;
+      const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+      const double pref = Kab * kq[8] * sqrt(inv);
+      double F[L + 1];
+      boys_f64<L>(T, boys, F);
+#pragma unroll
+      for (int m = 0; m <= L; ++m) F[m] *= pref;
+      C::template vrr<double>(F, PAx, PAy, PAz, -rop * PQx, -rop * PQy, -rop * PQz, QCx, QCy, QCz,
+                              roq * PQx, roq * PQy, roq * PQz, o2p, o2q, 0.5 * inv, rop, roq, acc);
+    }
+  }
+  C::template hrr<double>(acc, ABx, ABy, ABz, CDx, CDy, CDz, out);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// ------------------------------------------------------------ J/K kernel
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
+  constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  const int wid = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  if (wid >= a.ntasks) return;
+  const Task t = a.tasks[wid];
+  const int lane = threadIdx.x & 63;
+  const PairInfo& P = a.bra[t.bra];
+  const int j = t.j0 + lane;
+  bool active = j < t.j1;
+  PairInfo K;
+  if (active) K = a.ket[j];
+  else K = P;
+  const int n = a.n, ns = a.nshell;
+  if (active && a.screen) {
+    const double* Dn = a.Dn;
+    double dn = Dn[P.sa * ns + P.sb];
+    dn = fmax(dn, Dn[K.sa * ns + K.sb]);
+    dn = fmax(dn, Dn[P.sa * ns + K.sa]);
+    dn = fmax(dn, Dn[P.sb * ns + K.sb]);
+    dn = fmax(dn, Dn[P.sa * ns + K.sb]);
+    dn = fmax(dn, Dn[P.sb * ns + K.sa]);
+    if (dn * P.Q * K.Q < a.eps) active = false;
+  }
+  const uint64_t amask = __ballot(active);
+  if (a.counters) {
+    unsigned long long np = active ? (unsigned long long)P.nprim * (unsigned long long)K.nprim : 0ull;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) np += __shfl_xor(np, o, 64);
+    if (lane == 0) {
+      atomicAdd(&a.counters[0], (unsigned long long)__popcll(amask));
+      atomicAdd(&a.counters[1], np);
+    }
+  }
+  double jab[NA * NB];
+#pragma unroll
+  for (int k = 0; k < NA * NB; ++k) jab[k] = 0.0;
+  if (active) {
+    double out[NA * NB * NC * ND];
+    contracted_quartet<LA, LB, LC, LD>(a.prim + (size_t)P.poff * kPrimPairStride, P.nprim,
+                                       a.prim + (size_t)K.poff * kPrimPairStride, K.nprim, a.boys,
+                                       P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
+    const double deg = (P.sa != P.sb ? 2.0 : 1.0) * (K.sa != K.sb ? 2.0 : 1.0) *
+                       ((a.same_class && j == t.bra) ? 1.0 : 2.0);
+    const double* D = a.D;
+    const double* nrm = a.norma;
+    double na[NA], nb[NB], nc[NC], nd[ND];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) na[i] = nrm[P.bfa + i];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) nb[i] = nrm[P.bfb + i];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) nc[i] = nrm[K.bfa + i];
+#pragma unroll
+    for (int i = 0; i < ND; ++i) nd[i] = nrm[K.bfb + i];
+    // normalised density blocks
+    double dab[NA * NB], dcd[NC * ND], dac[NA * NC], dbd[NB * ND], dad[NA * ND], dbc[NB * NC];
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+#pragma unroll
+      for (int k = 0; k < NB; ++k) dab[i * NB + k] = D[(size_t)(P.bfa + i) * n + P.bfb + k] * na[i] * nb[k];
+#pragma unroll
+    for (int i = 0; i < NC; ++i)
+#pragma unroll
+      for (int k = 0; k < ND; ++k) dcd[i * ND + k] = D[(size_t)(K.bfa + i) * n + K.bfb + k] * nc[i] * nd[k];
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+#pragma unroll
+      for (int k = 0; k < NC; ++k) dac[i * NC + k] = D[(size_t)(P.bfa + i) * n + K.bfa + k] * na[i] * nc[k];
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+      for (int k = 0; k < ND; ++k) dbd[i * ND + k] = D[(size_t)(P.bfb + i) * n + K.bfb + k] * nb[i] * nd[k];
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+#pragma unroll
+      for (int k = 0; k < ND; ++k) dad[i * ND + k] = D[(size_t)(P.bfa + i) * n + K.bfb + k] * na[i] * nd[k];
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+      for (int k = 0; k < NC; ++k) dbc[i * NC + k] = D[(size_t)(P.bfb + i) * n + K.bfa + k] * nb[i] * nc[k];
+    double jcd[NC * ND], kac[NA * NC], kbd[NB * ND], kad[NA * ND], kbc[NB * NC];
+#pragma unroll
+    for (int k = 0; k < NC * ND; ++k) jcd[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NA * NC; ++k) kac[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NB * ND; ++k) kbd[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NA * ND; ++k) kad[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NB * NC; ++k) kbc[k] = 0.0;
+#pragma unroll
+    for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+      for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+        for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+          for (int id = 0; id < ND; ++id) {
+            const double v = out[((ia * NB + ib) * NC + ic) * ND + id];
+            jab[ia * NB + ib] += dcd[ic * ND + id] * v;
+            jcd[ic * ND + id] += dab[ia * NB + ib] * v;
+            kac[ia * NC + ic] += dbd[ib * ND + id] * v;
+            kbd[ib * ND + id] += dac[ia * NC + ic] * v;
+            kad[ia * ND + id] += dbc[ib * NC + ic] * v;
+            kbc[ib * NC + ic] += dad[ia * ND + id] * v;
+          }
+    const double sj = deg, sk = deg * a.half_factor;
+    double* G = a.Graw;
+#pragma unroll
+    for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+      for (int id = 0; id < ND; ++id)
+        unsafeAtomicAdd(&G[(size_t)(K.bfa + ic) * n + K.bfb + id], jcd[ic * ND + id] * sj * nc[ic] * nd[id]);
+#pragma unroll
+    for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+      for (int ic = 0; ic < NC; ++ic)
+        unsafeAtomicAdd(&G[(size_t)(P.bfa + ia) * n + K.bfa + ic], kac[ia * NC + ic] * sk * na[ia] * nc[ic]);
+#pragma unroll
+    for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+      for (int id = 0; id < ND; ++id)
+        unsafeAtomicAdd(&G[(size_t)(P.bfb + ib) * n + K.bfb + id], kbd[ib * ND + id] * sk * nb[ib] * nd[id]);
+#pragma unroll
+    for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+      for (int id = 0; id < ND; ++id)
+        unsafeAtomicAdd(&G[(size_t)(P.bfa + ia) * n + K.bfb + id], kad[ia * ND + id] * sk * na[ia] * nd[id]);
+#pragma unroll
+    for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+      for (int ic = 0; ic < NC; ++ic)
+        unsafeAtomicAdd(&G[(size_t)(P.bfb + ib) * n + K.bfa + ic], kbc[ib * NC + ic] * sk * nb[ib] * nc[ic]);
+#pragma unroll
+    for (int k = 0; k < NA * NB; ++k) jab[k] *= deg;
+  }
+  // the bra pair is shared by the whole wavefront: reduce its J block
+#pragma unroll
+  for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+    for (int ib = 0; ib < NB; ++ib) {
+      const double s = wave_sum(jab[ia * NB + ib]);
+      if (lane == 0 && amask)
+        unsafeAtomicAdd(&a.Graw[(size_t)(P.bfa + ia) * n + P.bfb + ib],
+                        s * a.norma[P.bfa + ia] * a.norma[P.bfb + ib]);
+    }
+}
+
+// ------------------------------------------------------- coupling kernel
+// Quartets (bra pair of species X | ket pair of species Y) over ALL pairs
+// s1 <= s2 of both species.  Computes the result on the bra species from the
+// ket species' density (rbra) and/or on the ket species from the bra
+// species' density (rket), each with the reference's element sets and drop.
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(256) void coupling_kernel(CouplingArgs a) {
+  constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  const int wid = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  if (wid >= a.ntasks) return;
+  const Task t = a.tasks[wid];
+  const int lane = threadIdx.x & 63;
+  const int j = t.j0 + lane;
+  if (j >= t.j1) return;
+  const PairInfo& P = a.bra[t.bra];
+  const PairInfo K = a.ket[j];
+  double out[NA * NB * NC * ND];
+  contracted_quartet<LA, LB, LC, LD>(a.bprim + (size_t)P.poff * kPrimPairStride, P.nprim,
+                                     a.kprim + (size_t)K.poff * kPrimPairStride, K.nprim, a.boys,
+                                     P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
+  const double drop = a.drop;
+#pragma unroll
+  for (int k = 0; k < NA * NB * NC * ND; ++k)
+    if (fabs(out[k]) < drop) out[k] = 0.0;
+  const bool bdiag = P.sa == P.sb, kdiag = K.sa == K.sb;
+  if (a.rbra) {
+    // weights of the ket density: element set of0 <= of1 within a diagonal
+    // shell pair, doubled off the function diagonal; normalised
+    double w[NC * ND];
+#pragma unroll
+    for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+      for (int id = 0; id < ND; ++id) {
+        const int fc = K.bfa + ic, fd = K.bfb + id;
+        double v = 0.0;
+        if (!kdiag || ic <= id)
+          v = (fc != fd ? 2.0 : 1.0) * a.dket[(size_t)fc * a.nket + fd] * a.nket_norma[fc] * a.nket_norma[fd];
+        w[ic * ND + id] = v;
+      }
+#pragma unroll
+    for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+      for (int ib = 0; ib < NB; ++ib) {
+        if (bdiag && ia > ib) continue;
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < NC * ND; ++k) s += w[k] * out[(ia * NB + ib) * NC * ND + k];
+        const int fa = P.bfa + ia, fb = P.bfb + ib;
+        const int r = fa > fb ? fa : fb, c = fa > fb ? fb : fa;
+        unsafeAtomicAdd(&a.rbra[(size_t)r * a.nbra + c], s * a.nbra_norma[fa] * a.nbra_norma[fb]);
+      }
+  }
+  if (a.rket) {
+    double w[NA * NB];
+#pragma unroll
+    for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+      for (int ib = 0; ib < NB; ++ib) {
+        const int fa = P.bfa + ia, fb = P.bfb + ib;
+        double v = 0.0;
+        if (!bdiag || ia <= ib)
+          v = (fa != fb ? 2.0 : 1.0) * a.dbra[(size_t)fa * a.nbra + fb] * a.nbra_norma[fa] * a.nbra_norma[fb];
+        w[ia * NB + ib] = v;
+      }
+#pragma unroll
+    for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+      for (int id = 0; id < ND; ++id) {
+        if (kdiag && ic > id) continue;
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < NA * NB; ++k) s += w[k] * out[k * NC * ND + ic * ND + id];
+        const int fc = K.bfa + ic, fd = K.bfb + id;
+        const int r = fc > fd ? fc : fd, c = fc > fd ? fd : fc;
+        unsafeAtomicAdd(&a.rket[(size_t)r * a.nket + c], s * a.nket_norma[fc] * a.nket_norma[fd]);
+      }
+  }
+}
+
+// --------------------------------------------------- Schwarz (a11) kernel
+// Q(pair) = sqrt(max_i |(ij|ij)_raw|) over the diagonal of the pair's
+// (ab|ab) block, no primitive screening (Libint2Iface.cpp:3361-3404).
+template <int LA, int LB>
+__global__ __launch_bounds__(256) void schwarz_kernel(PairInfo* pairs, int npairs, const double* prim,
+                                                      const double* boys) {
+  constexpr int NA = ncart(LA), NB = ncart(LB), N2 = NA * NB;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npairs) return;
+  PairInfo P = pairs[i];
+  double out[N2 * N2];
+  const double* pp = prim + (size_t)P.poff * kPrimPairStride;
+  contracted_quartet<LA, LB, LA, LB>(pp, P.nprim, pp, P.nprim, boys, P.ABx, P.ABy, P.ABz, P.ABx,
+                                     P.ABy, P.ABz, out);
+  double mx = 0.0;
+#pragma unroll
+  for (int k = 0; k < N2; ++k) mx = fmax(mx, fabs(out[k * N2 + k]));
+  pairs[i].Q = sqrt(mx);
+}
+
+template <int LA, int LB, int LC, int LD>
+static void launch_jk_t(const JKArgs& a, hipStream_t s) {
+  if (a.ntasks <= 0) return;
+  const int blocks = (a.ntasks + 3) / 4;
+  hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD>), dim3(blocks), dim3(256), 0, s, a);
+}
+
+template <int LA, int LB, int LC, int LD>
+static void launch_cp_t(const CouplingArgs& a, hipStream_t s) {
+  if (a.ntasks <= 0) return;
+  const int blocks = (a.ntasks + 3) / 4;
+  hipLaunchKernelGGL((coupling_kernel<LA, LB, LC, LD>), dim3(blocks), dim3(256), 0, s, a);
+}
+
+template <int LA, int LB>
+static void launch_schwarz_t(PairInfo* pairs, int npairs, const double* prim, const double* boys, hipStream_t s) {
+  if (npairs <= 0) return;
+  hipLaunchKernelGGL((schwarz_kernel<LA, LB>), dim3((npairs + 255) / 256), dim3(256), 0, s, pairs, npairs, prim, boys);
+}
+
+}  // namespace lowdin

@@ -1,0 +1,291 @@
+"""Generate class-specialised Obara-Saika VRR and Head-Gordon-Pople HRR code.
+
+For every canonical quartet class (la lb|lc ld) with l <= LMAX, la >= lb,
+lc >= ld and pair class (la, lb) >= (lc, ld), this emits two device
+functions into ``gen/eri_<class>.inc``, a kernel translation unit
+``gen/k_<class>.hip`` per class (so the classes compile in parallel), and the
+launch dispatch table ``gen/dispatch.h``:
+
+``vrr_<la><lb><lc><ld>(F, ...)``
+    Obara-Saika vertical recursion (Obara & Saika, J. Chem. Phys. 84, 3963
+    (1986), eqs. 39 and its ket analogue) from the scaled Boys values
+    F[m] = pref * F_m(T) to the targets [e0|f0]^(0), |e| in [la, la+lb],
+    |f| in [lc, lc+ld], added into the contracted accumulator ``acc``.
+    Every intermediate is a named local, emitted once in dependency order.
+
+``hrr_<la><lb><lc><ld>(acc, AB, CD, out)``
+    Head-Gordon-Pople horizontal recursion (J. Chem. Phys. 89, 5777 (1988)):
+    (a, b+1_i| = (a+1_i, b| + AB_i (a b| on the bra, then the same on the ket
+    with CD, producing (ab|cd) in the standard Cartesian order (xx, xy, xz,
+    yy, yz, zz), index ((ia*nb + ib)*nc + ic)*nd + id.
+
+Also emits ``vrr_flops`` / ``hrr_flops`` tables (floating-point operations
+per primitive quartet and per contracted quartet) used by the roofline
+accounting.
+
+Run: python openlowdin_amd/csrc/gen_eri.py  (the package build does this).
+"""
+import itertools
+import json
+import os
+import sys
+
+LMAX = 2
+XYZ = "xyz"
+
+
+def comps(l):
+    return [(lx, ly, l - lx - ly) for lx in range(l, -1, -1) for ly in range(l - lx, -1, -1)]
+
+
+def ncart(l):
+    return (l + 1) * (l + 2) // 2
+
+
+def cidx(v):
+    return comps(sum(v)).index(tuple(v))
+
+
+def add(v, i, d):
+    w = list(v)
+    w[i] += d
+    return tuple(w)
+
+
+def pick(v):
+    """Direction to reduce: the largest component (ties: x, then y, then z)."""
+    best = 0
+    for i in range(3):
+        if v[i] > v[best]:
+            best = i
+    return best
+
+
+def canonical_classes(lmax=LMAX):
+    pcs = [(a, b) for a in range(lmax + 1) for b in range(a + 1)]
+    pcs.sort(key=lambda p: (p[0] + p[1], p[0]))
+    out = []
+    for i, X in enumerate(pcs):
+        for Y in pcs[: i + 1]:
+            out.append((X[0], X[1], Y[0], Y[1]))
+    return pcs, out
+
+
+class VRRGen:
+    def __init__(self, la, lb, lc, ld):
+        self.la, self.lb, self.lc, self.ld = la, lb, lc, ld
+        self.lines = []
+        self.memo = {}
+        self.nv = 0
+        self.flops = 0
+        self.es = [c for l in range(la, la + lb + 1) for c in comps(l)]
+        self.fs = [c for l in range(lc, lc + ld + 1) for c in comps(l)]
+
+    def new(self, expr, flops):
+        name = "v%d" % self.nv
+        self.nv += 1
+        self.lines.append("  const R %s = %s;" % (name, expr))
+        self.flops += flops
+        return name
+
+    def need(self, e, f, m):
+        if min(e) < 0 or min(f) < 0:
+            return None
+        key = (e, f, m)
+        if key in self.memo:
+            return self.memo[key]
+        if sum(e) == 0 and sum(f) == 0:
+            name = "F[%d]" % m
+        elif sum(f) > 0:
+            i = pick(f)
+            f1 = add(f, i, -1)
+            t = [("QC%s" % XYZ[i], self.need(e, f1, m)), ("WQ%s" % XYZ[i], self.need(e, f1, m + 1))]
+            terms = ["%s*%s" % (c, v) for c, v in t if v is not None]
+            fl = 2 * len(terms) - 1
+            if f1[i] > 0:
+                f2 = add(f1, i, -1)
+                a, b = self.need(e, f2, m), self.need(e, f2, m + 1)
+                terms.append("(R)%d*o2q*(%s - roq*%s)" % (f1[i], a, b))
+                fl += 5
+            if e[i] > 0:
+                e1 = add(e, i, -1)
+                c = self.need(e1, f1, m + 1)
+                terms.append("(R)%d*o2pq*%s" % (e[i], c))
+                fl += 3
+            name = self.new(" + ".join(terms), fl)
+        else:
+            i = pick(e)
+            e1 = add(e, i, -1)
+            t = [("PA%s" % XYZ[i], self.need(e1, f, m)), ("WP%s" % XYZ[i], self.need(e1, f, m + 1))]
+            terms = ["%s*%s" % (c, v) for c, v in t if v is not None]
+            fl = 2 * len(terms) - 1
+            if e1[i] > 0:
+                e2 = add(e1, i, -1)
+                a, b = self.need(e2, f, m), self.need(e2, f, m + 1)
+                terms.append("(R)%d*o2p*(%s - rop*%s)" % (e1[i], a, b))
+                fl += 5
+            name = self.new(" + ".join(terms), fl)
+        self.memo[key] = name
+        return name
+
+    def emit(self):
+        body = []
+        nf = len(self.fs)
+        for ie, e in enumerate(self.es):
+            for jf, f in enumerate(self.fs):
+                v = self.need(e, f, 0)
+                body.append("  acc[%d] += %s;" % (ie * nf + jf, v))
+                self.flops += 1
+        L = self.la + self.lb + self.lc + self.ld
+        sig = ("template <typename R>\n__device__ __forceinline__ void vrr_%d%d%d%d(const R* __restrict__ F, "
+               "R PAx, R PAy, R PAz, R WPx, R WPy, R WPz, R QCx, R QCy, R QCz, R WQx, R WQy, R WQz, "
+               "R o2p, R o2q, R o2pq, R rop, R roq, R* __restrict__ acc) {" % (self.la, self.lb, self.lc, self.ld))
+        unused = "  (void)PAx; (void)PAy; (void)PAz; (void)WPx; (void)WPy; (void)WPz; (void)QCx; (void)QCy; (void)QCz;\n" \
+                 "  (void)WQx; (void)WQy; (void)WQz; (void)o2p; (void)o2q; (void)o2pq; (void)rop; (void)roq;"
+        return "\n".join([sig, unused] + self.lines + body + ["}"]), L
+
+
+class HRRGen:
+    def __init__(self, la, lb, lc, ld):
+        self.la, self.lb, self.lc, self.ld = la, lb, lc, ld
+        self.lines = []
+        self.memo = {}
+        self.nv = 0
+        self.flops = 0
+        self.es = [c for l in range(la, la + lb + 1) for c in comps(l)]
+        self.fs = [c for l in range(lc, lc + ld + 1) for c in comps(l)]
+        self.eidx = {e: k for k, e in enumerate(self.es)}
+        self.fidx = {f: k for k, f in enumerate(self.fs)}
+
+    def new(self, expr):
+        name = "h%d" % self.nv
+        self.nv += 1
+        self.lines.append("  const R %s = %s;" % (name, expr))
+        self.flops += 2
+        return name
+
+    def bra(self, a, b, f):
+        key = ("b", a, b, f)
+        if key in self.memo:
+            return self.memo[key]
+        if sum(b) == 0:
+            name = "acc[%d]" % (self.eidx[a] * len(self.fs) + self.fidx[f])
+        else:
+            i = pick(b)
+            b1 = add(b, i, -1)
+            x = self.bra(add(a, i, 1), b1, f)
+            y = self.bra(a, b1, f)
+            name = self.new("%s + AB%s*%s" % (x, XYZ[i], y))
+        self.memo[key] = name
+        return name
+
+    def ket(self, a, b, c, d):
+        key = ("k", a, b, c, d)
+        if key in self.memo:
+            return self.memo[key]
+        if sum(d) == 0:
+            name = self.bra(a, b, c)
+        else:
+            i = pick(d)
+            d1 = add(d, i, -1)
+            x = self.ket(a, b, add(c, i, 1), d1)
+            y = self.ket(a, b, c, d1)
+            name = self.new("%s + CD%s*%s" % (x, XYZ[i], y))
+        self.memo[key] = name
+        return name
+
+    def emit(self):
+        out = []
+        la, lb, lc, ld = self.la, self.lb, self.lc, self.ld
+        nb, nc, nd = ncart(lb), ncart(lc), ncart(ld)
+        for ia, a in enumerate(comps(la)):
+            for ib, b in enumerate(comps(lb)):
+                for ic, c in enumerate(comps(lc)):
+                    for id_, d in enumerate(comps(ld)):
+                        v = self.ket(a, b, c, d)
+                        out.append("  out[%d] = %s;" % (((ia * nb + ib) * nc + ic) * nd + id_, v))
+        sig = ("template <typename R>\n__device__ __forceinline__ void hrr_%d%d%d%d(const R* __restrict__ acc, "
+               "R ABx, R ABy, R ABz, R CDx, R CDy, R CDz, R* __restrict__ out) {" % (la, lb, lc, ld))
+        unused = "  (void)ABx; (void)ABy; (void)ABz; (void)CDx; (void)CDy; (void)CDz;"
+        return "\n".join([sig, unused] + self.lines + out + ["}"])
+
+
+KERNEL_TU = """// Generated by gen_eri.py -- do not edit.
+// Kernels of quartet class (%(a)d%(b)d|%(c)d%(d)d): fused ERI + J/K, coupling%(schw)s.
+#include "../eri_core.h"
+
+namespace lowdin {
+#include "eri_%(k)s.inc"
+LOWDIN_ERI_CLASS(%(a)d, %(b)d, %(c)d, %(d)d)
+
+void launch_jk_%(k)s(const JKArgs& args, hipStream_t s) { launch_jk_t<%(a)d, %(b)d, %(c)d, %(d)d>(args, s); }
+void launch_cp_%(k)s(const CouplingArgs& args, hipStream_t s) { launch_cp_t<%(a)d, %(b)d, %(c)d, %(d)d>(args, s); }
+%(schwfn)s}  // namespace lowdin
+"""
+
+
+def main(outdir):
+    pcs, classes = canonical_classes()
+    gdir = os.path.join(outdir, "gen")
+    os.makedirs(gdir, exist_ok=True)
+    table = {}
+    for (la, lb, lc, ld) in classes:
+        k = "%d%d%d%d" % (la, lb, lc, ld)
+        v = VRRGen(la, lb, lc, ld)
+        code, L = v.emit()
+        h = HRRGen(la, lb, lc, ld)
+        hcode = h.emit()
+        nacc = len(v.es) * len(v.fs)
+        inc = ["// Generated by gen_eri.py -- do not edit.",
+               "// class (%d%d|%d%d): L = %d, %d accumulators, VRR %d flops/primitive quartet, HRR %d flops"
+               % (la, lb, lc, ld, L, nacc, v.flops, h.flops),
+               "#define NACC_%s %d" % (k, nacc), code, hcode, ""]
+        with open(os.path.join(gdir, "eri_%s.inc" % k), "w") as fh:
+            fh.write("\n".join(inc))
+        diag = (la, lb) == (lc, ld)
+        schwfn = ("void launch_schwarz_%d%d(PairInfo* pairs, int npairs, const double* prim, const double* boys, "
+                  "hipStream_t s) { launch_schwarz_t<%d, %d>(pairs, npairs, prim, boys, s); }\n" % (la, lb, la, lb)
+                  if diag else "")
+        with open(os.path.join(gdir, "k_%s.hip" % k), "w") as fh:
+            fh.write(KERNEL_TU % {"a": la, "b": lb, "c": lc, "d": ld, "k": k,
+                                  "schw": ", Schwarz" if diag else "", "schwfn": schwfn})
+        table[k] = {"L": L, "nacc": nacc, "vrr_flops": v.flops, "hrr_flops": h.flops}
+    with open(os.path.join(outdir, "ops_table.json"), "w") as fh:
+        json.dump({"note": "floating-point operations of the generated Obara-Saika VRR (per primitive "
+                           "quartet, excluding Boys) and HGP HRR (per contracted quartet)",
+                   "classes": table}, fh, indent=1)
+    # dispatch table
+    lines = ["// Generated by gen_eri.py -- do not edit.", "#pragma once", '#include "../lowdin_kernels.h"',
+             "namespace lowdin {"]
+    for (la, lb, lc, ld) in classes:
+        k = "%d%d%d%d" % (la, lb, lc, ld)
+        lines.append("void launch_jk_%s(const JKArgs&, hipStream_t);" % k)
+        lines.append("void launch_cp_%s(const CouplingArgs&, hipStream_t);" % k)
+    for (la, lb) in pcs:
+        lines.append("void launch_schwarz_%d%d(PairInfo*, int, const double*, const double*, hipStream_t);" % (la, lb))
+    lines.append("inline bool dispatch_jk(int la, int lb, int lc, int ld, const JKArgs& a, hipStream_t s) {")
+    lines.append("  switch (la * 1000 + lb * 100 + lc * 10 + ld) {")
+    for (la, lb, lc, ld) in classes:
+        lines.append("    case %d: launch_jk_%d%d%d%d(a, s); return true;" % (la * 1000 + lb * 100 + lc * 10 + ld, la, lb, lc, ld))
+    lines.append("    default: return false;\n  }\n}")
+    lines.append("inline bool dispatch_cp(int la, int lb, int lc, int ld, const CouplingArgs& a, hipStream_t s) {")
+    lines.append("  switch (la * 1000 + lb * 100 + lc * 10 + ld) {")
+    for (la, lb, lc, ld) in classes:
+        lines.append("    case %d: launch_cp_%d%d%d%d(a, s); return true;" % (la * 1000 + lb * 100 + lc * 10 + ld, la, lb, lc, ld))
+    lines.append("    default: return false;\n  }\n}")
+    lines.append("inline bool dispatch_schwarz(int la, int lb, PairInfo* p, int np, const double* prim, const double* boys, hipStream_t s) {")
+    lines.append("  switch (la * 10 + lb) {")
+    for (la, lb) in pcs:
+        lines.append("    case %d: launch_schwarz_%d%d(p, np, prim, boys, s); return true;" % (la * 10 + lb, la, lb))
+    lines.append("    default: return false;\n  }\n}")
+    lines.append("}  // namespace lowdin")
+    with open(os.path.join(gdir, "dispatch.h"), "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    return table
+
+
+if __name__ == "__main__":
+    d = sys.argv[1] if len(sys.argv) > 1 else os.path.dirname(os.path.abspath(__file__))
+    t = main(d)
+    for k, v in t.items():
+        print(k, v)

@@ -1,0 +1,100 @@
+// Kernel argument blocks and launchers shared by the host library and the
+// HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "lowdin_device.h"
+
+namespace lowdin {
+
+struct JKArgs {
+  const Task* tasks;
+  int ntasks;
+  const PairInfo* bra;
+  const PairInfo* ket;
+  const double* prim;      // PrimPair records of this species
+  const double* D;         // n x n density
+  const double* Dn;        // nshell x nshell block infinity norms
+  const double* norma;
+  const double* boys;
+  double* Graw;            // n x n raw accumulator (symmetrised afterwards)
+  unsigned long long* counters;  // [0] computed quartets, [1] primitive quartets (optional)
+  int n;
+  int nshell;
+  int screen;              // 1: Dnorm*Q*Q < eps screen (electrons)
+  int same_class;          // bra and ket lists are the same class list
+  double eps;
+  double half_factor;      // 0.5 * exchange factor
+};
+
+struct CouplingArgs {
+  const Task* tasks;
+  int ntasks;
+  const PairInfo* bra;     // pairs of the bra species
+  const PairInfo* ket;     // pairs of the ket species
+  const double* bprim;
+  const double* kprim;
+  const double* nbra_norma;
+  const double* nket_norma;
+  const double* dbra;      // density of the bra species (for rket)
+  const double* dket;      // density of the ket species (for rbra)
+  double* rbra;            // result on the bra species (may be null)
+  double* rket;            // result on the ket species (may be null)
+  const double* boys;
+  int nbra;
+  int nket;
+  double drop;             // raw-integral drop (1e-10)
+};
+
+// Raw overlap block <A|B> (row-major, A's components slowest) of two shells
+// with l <= 2, from primitives whose coefficients already carry N(alpha,l).
+// Obara-Saika 1-D recursion S(i+1,j) = PA S(i,j) + (i S(i-1,j) + j S(i,j-1))/(2p).
+__host__ __device__ inline void overlap_block(const ShellInfo& A, const ShellInfo& B, const double* pexp,
+                                              const double* pcoef, double* blk) {
+  const int la = A.l, lb = B.l;
+  const int na = (la + 1) * (la + 2) / 2, nb = (lb + 1) * (lb + 2) / 2;
+  int ca[6][3], cb[6][3];
+  {
+    int k = 0;
+    for (int lx = la; lx >= 0; --lx)
+      for (int ly = la - lx; ly >= 0; --ly) { ca[k][0] = lx; ca[k][1] = ly; ca[k][2] = la - lx - ly; ++k; }
+    k = 0;
+    for (int lx = lb; lx >= 0; --lx)
+      for (int ly = lb - lx; ly >= 0; --ly) { cb[k][0] = lx; cb[k][1] = ly; cb[k][2] = lb - lx - ly; ++k; }
+  }
+  for (int k = 0; k < na * nb; ++k) blk[k] = 0.0;
+  const double Ax[3] = {A.x, A.y, A.z}, Bx[3] = {B.x, B.y, B.z};
+  double ab2 = 0.0;
+  for (int d = 0; d < 3; ++d) ab2 += (Ax[d] - Bx[d]) * (Ax[d] - Bx[d]);
+  for (int i = 0; i < A.nprim; ++i) {
+    const double a = pexp[A.poff + i], ca_ = pcoef[A.poff + i];
+    for (int j = 0; j < B.nprim; ++j) {
+      const double b = pexp[B.poff + j], cb_ = pcoef[B.poff + j];
+      const double p = a + b, o2p = 0.5 / p;
+      const double pre = ca_ * cb_ * exp(-a * b / p * ab2) * pow(3.14159265358979323846 / p, 1.5);
+      double S[3][3][3];  // [axis][i][j], i, j <= 2
+      for (int d = 0; d < 3; ++d) {
+        const double P = (a * Ax[d] + b * Bx[d]) / p;
+        const double PA = P - Ax[d], PB = P - Bx[d];
+        S[d][0][0] = 1.0;
+        for (int ii = 0; ii < la; ++ii)
+          S[d][ii + 1][0] = PA * S[d][ii][0] + (ii > 0 ? ii * o2p * S[d][ii - 1][0] : 0.0);
+        for (int jj = 0; jj < lb; ++jj)
+          for (int ii = 0; ii <= la; ++ii)
+            S[d][ii][jj + 1] = PB * S[d][ii][jj] + (ii > 0 ? ii * o2p * S[d][ii - 1][jj] : 0.0) +
+                               (jj > 0 ? jj * o2p * S[d][ii][jj - 1] : 0.0);
+      }
+      for (int x = 0; x < na; ++x)
+        for (int y = 0; y < nb; ++y)
+          blk[x * nb + y] += pre * S[0][ca[x][0]][cb[y][0]] * S[1][ca[x][1]][cb[y][1]] * S[2][ca[x][2]][cb[y][2]];
+    }
+  }
+}
+
+void launch_pair_norms(const ShellInfo* sh, const double* pexp, const double* pcoef, int nsh, double* norms,
+                       hipStream_t s);
+void launch_dnorm(const double* D, int n, const ShellInfo* sh, int nsh, double* Dn, hipStream_t s);
+void launch_symmetrize(const double* Graw, int n, double scale, double* G, hipStream_t s);
+
+}  // namespace lowdin

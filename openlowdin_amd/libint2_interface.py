@@ -1,0 +1,212 @@
+"""Python mirror of openLOWDIN's Fortran-side interface to the direct Fock
+build, over the C ABI of liblowdin_fock.so.
+
+* :class:`Libint2Interface` mirrors ``type(Libint2Interface)`` and its
+  constructor / destructor / direct wrappers
+  (``src/ints/Libint2Interface.f90:36-39, 382-460, 514-547, 637-675``):
+  one handle per species, shells added in particle order then contraction
+  order with raw coefficients, the pair list rebuilt before every intra
+  build, result arrays allocated by the caller side.
+* :func:`get_direct_intra_repulsion_matrix` and
+  :func:`get_direct_inter_repulsion_matrix` mirror
+  ``DirectIntegralManager_getDirect{Intra,Inter}RepulsionMatrix``
+  (``src/ints/DirectIntegralManager.f90:66-179``), including the STOP on an
+  integral scheme other than LIBINT (raised here as ValueError).
+
+Matrices cross the boundary as Fortran column-major float64 arrays; the
+library is loaded from the package directory and its absence is an error
+(there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, List, Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblowdin_fock.so")
+_LIB = None
+
+
+class LowdinFockError(RuntimeError):
+    pass
+
+
+class Stats(C.Structure):
+    _fields_ = [("quartets", C.c_uint64), ("eris", C.c_uint64), ("prim_quartets", C.c_uint64),
+                ("kernel_ms", C.c_double), ("total_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def lib():
+    """Load liblowdin_fock.so (built by ``__graft_entry__.build()``)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise LowdinFockError("liblowdin_fock.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
+        L.LibintInterface_new.argtypes = [C.c_int, C.c_int, C.c_bool, C.c_bool]
+        L.LibintInterface_new.restype = vp
+        L.LibintInterface_del.argtypes = [vp]
+        L.LibintInterface_add_particle.argtypes = [vp, C.c_int, dp]
+        L.LibintInterface_add_shell.argtypes = [vp, dp, dp, dp, C.c_int, C.c_int]
+        L.LibintInterface_get_nbasis.argtypes = [vp]
+        L.LibintInterface_get_nbasis.restype = C.c_int
+        L.LibintInterface_init_2body_ints.argtypes = [vp]
+        L.LibintInterface_compute_2body_direct.argtypes = [vp, dp, dp, dp]
+        L.LibintInterface_compute_coupling_direct.argtypes = [vp, vp, dp, dp]
+        L.lowdin_fock_last_error.restype = C.c_char_p
+        L.lowdin_fock_get_stats.argtypes = [vp, C.POINTER(Stats)]
+        L.lowdin_fock_shellpair_list.argtypes = [vp, ip, ip, C.c_int]
+        L.lowdin_fock_shellpair_list.restype = C.c_int
+        L.lowdin_fock_schwarz.argtypes = [vp, dp]
+        L.lowdin_fock_schwarz.restype = C.c_int
+        L.lowdin_fock_coupling_pair.argtypes = [vp, vp, dp, dp, dp, dp]
+        L.lowdin_fock_coupling_pair.restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def _dptr(a: Optional[np.ndarray]):
+    if a is None:
+        return C.cast(None, C.POINTER(C.c_double))
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _check(where: str):
+    msg = lib().lowdin_fock_last_error()
+    if msg:
+        raise LowdinFockError("%s: %s" % (where, msg.decode()))
+
+
+def _fortran(m: np.ndarray, n: int) -> np.ndarray:
+    a = np.asfortranarray(m, dtype=np.float64)
+    if a.shape != (n, n):
+        raise ValueError("expected an %d x %d matrix, got %s" % (n, n, a.shape))
+    return a
+
+
+class Libint2Interface:
+    """One species' handle (``type(Libint2Interface)``)."""
+
+    def __init__(self, species, species_id: int, stack_size: int = 30000, parallel: bool = True):
+        self.species = species
+        self.species_id = species_id
+        self.this = lib().LibintInterface_new(stack_size, species_id, bool(species.is_electron), bool(parallel))
+        if not self.this:
+            raise LowdinFockError("LibintInterface_new returned NULL")
+        self._keep = []
+        # Libint2Interface_constructor: per particle add_particle, then the
+        # particle's contractions (Libint2Interface.f90:412-446).  The species
+        # model here stores shells already in that order.
+        for sh in species.shells:
+            e = np.ascontiguousarray(sh.exps, dtype=np.float64)
+            c = np.ascontiguousarray(sh.coefs, dtype=np.float64)
+            o = np.ascontiguousarray(sh.origin, dtype=np.float64)
+            lib().LibintInterface_add_shell(self.this, _dptr(e), _dptr(c), _dptr(o), int(sh.l), len(e))
+            _check("add_shell")
+        self.nbasis = lib().LibintInterface_get_nbasis(self.this)
+        self.is_instanced = True
+
+    def close(self):
+        if getattr(self, "this", None):
+            lib().LibintInterface_del(self.this)
+            self.this = None
+            self.is_instanced = False
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # Libint2Interface_compute2BodyIntraspecies_direct (Libint2Interface.f90:514-547)
+    def compute_2body_intraspecies_direct(self, density: np.ndarray, factor: float) -> np.ndarray:
+        n = self.nbasis
+        D = _fortran(density, n)
+        G = np.zeros((n, n), dtype=np.float64, order="F")
+        f = C.c_double(factor)
+        lib().LibintInterface_init_2body_ints(self.this)
+        _check("init_2body_ints")
+        lib().LibintInterface_compute_2body_direct(self.this, _dptr(D), _dptr(G), C.byref(f))
+        _check("compute_2body_direct")
+        return G
+
+    # Libint2Interface_compute2BodyInterspecies_direct (Libint2Interface.f90:637-675)
+    def compute_2body_interspecies_direct(self, other: "Libint2Interface", density_other: np.ndarray) -> np.ndarray:
+        n = self.nbasis
+        D = _fortran(density_other, other.nbasis)
+        B = np.zeros((n, n), dtype=np.float64, order="F")
+        lib().LibintInterface_compute_coupling_direct(self.this, other.this, _dptr(D), _dptr(B))
+        _check("compute_coupling_direct")
+        return B
+
+    def coupling_pair(self, other: "Libint2Interface", d_self: np.ndarray, d_other: np.ndarray):
+        """Both coupling directions from one pass over the quartets."""
+        Da = _fortran(d_self, self.nbasis)
+        Db = _fortran(d_other, other.nbasis)
+        Ra = np.zeros((self.nbasis, self.nbasis), order="F")
+        Rb = np.zeros((other.nbasis, other.nbasis), order="F")
+        rc = lib().lowdin_fock_coupling_pair(self.this, other.this, _dptr(Da), _dptr(Db), _dptr(Ra), _dptr(Rb))
+        if rc != 0:
+            _check("coupling_pair")
+            raise LowdinFockError("coupling_pair failed")
+        return Ra, Rb
+
+    def stats(self) -> dict:
+        s = Stats()
+        lib().lowdin_fock_get_stats(self.this, C.byref(s))
+        return s.as_dict()
+
+    def shellpair_list(self) -> np.ndarray:
+        ns = len(self.species.shells)
+        cap = ns * (ns + 1) // 2
+        s1 = np.zeros(cap, dtype=np.int32)
+        s2 = np.zeros(cap, dtype=np.int32)
+        n = lib().lowdin_fock_shellpair_list(self.this, s1.ctypes.data_as(C.POINTER(C.c_int)),
+                                             s2.ctypes.data_as(C.POINTER(C.c_int)), cap)
+        if n < 0:
+            raise LowdinFockError("pair list not built (call init_2body_ints first)")
+        return np.stack([s1[:n], s2[:n]], axis=1)
+
+    def init_2body_ints(self):
+        lib().LibintInterface_init_2body_ints(self.this)
+        _check("init_2body_ints")
+
+    def schwarz(self) -> np.ndarray:
+        ns = len(self.species.shells)
+        Q = np.zeros(ns * ns)
+        if lib().lowdin_fock_schwarz(self.this, _dptr(Q)) != 0:
+            _check("schwarz")
+        return Q.reshape(ns, ns)
+
+
+# ------------------------------------------------ DirectIntegralManager
+def _instance(objs: Dict[int, Libint2Interface], system, species_id: int) -> Libint2Interface:
+    if species_id not in objs or not objs[species_id].is_instanced:
+        objs[species_id] = Libint2Interface(system.species[species_id - 1], species_id)
+    return objs[species_id]
+
+
+def get_direct_intra_repulsion_matrix(species_id: int, scheme: str, density: np.ndarray, factor: float,
+                                      system, objs: Dict[int, Libint2Interface]) -> np.ndarray:
+    """DirectIntegralManager_getDirectIntraRepulsionMatrix (DirectIntegralManager.f90:66-130).
+    species_id is 1-based, as in the reference."""
+    if scheme.strip().upper() != "LIBINT":
+        raise ValueError("The integral method selected has not been implemented")
+    return _instance(objs, system, species_id).compute_2body_intraspecies_direct(np.array(density), factor)
+
+
+def get_direct_inter_repulsion_matrix(species_id: int, other_species_id: int, scheme: str, density: np.ndarray,
+                                      system, objs: Dict[int, Libint2Interface]) -> np.ndarray:
+    """DirectIntegralManager_getDirectInterRepulsionMatrix (DirectIntegralManager.f90:139-179)."""
+    if scheme.strip().upper() != "LIBINT":
+        raise ValueError("The integral method selected has not been implemented")
+    a = _instance(objs, system, species_id)
+    b = _instance(objs, system, other_species_id)
+    return a.compute_2body_interspecies_direct(b, np.array(density))
