@@ -105,6 +105,22 @@ int lowdin_fock_schwarz(void* lint, double* Q);
  * result may be NULL.  Returns 0 on success. */
 int lowdin_fock_coupling_pair(void* a, void* b, const double* da, const double* db, double* ra, double* rb);
 
+/* Device-resident variant of LibintInterface_compute_2body_direct: d_dens
+ * and d_result are device pointers (n x n FP64); the build is enqueued on
+ * `stream` (a hipStream_t; NULL = the library's stream).  With synchronize
+ * != 0 the call waits and updates the statistics.  Returns 0 on success. */
+int lowdin_fock_compute_2body_device(void* lint, const double* d_dens, double* d_result, double factor,
+                                     void* stream, int synchronize);
+
+/* Record HIP events around every class launch of the intra build. */
+int lowdin_fock_time_launches(void* lint, int enable);
+
+/* Per class launch of the last intra build: class code 1000*la+100*lb+10*lc+ld,
+ * device ms (when timing is on), shell quartets and primitive quartets.
+ * Returns the number of launches. */
+int lowdin_fock_launch_profile(void* lint, int cap, int* classes, float* ms, uint64_t* quartets,
+                               uint64_t* prim_quartets);
+
 #ifdef __cplusplus
 }
 #endif

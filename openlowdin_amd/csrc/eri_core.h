@@ -30,6 +30,12 @@ namespace lowdin {
 
 __host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
 
+// 1/sqrt(x) for x > 0: hardware v_rsq_f64 refined by one Newton step.
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double r = __builtin_amdgcn_rsq(x);
+  return r * (1.5 - 0.5 * x * r * r);
+}
+
 // -------------------------------------------------------------- Boys F_m(T)
 // Taylor expansion of F_L around the nearest tabulated point, exp(-T) the
 // same way, then downward recursion F_m = (2T F_{m+1} + e^{-T}) / (2m+1).
@@ -65,7 +71,7 @@ __device__ __forceinline__ void boys_f64(double T, const double* __restrict__ ta
   } else {
     // erf(sqrt(T)) = 1 to 4e-19 here; the upward recursion keeps its
     // exp(-T) term, which is not negligible against F_m for m ~ T/5.
-    F[0] = 0.88622692545275801365 * rsqrt(T);
+    F[0] = 0.88622692545275801365 * rsqrt_nr(T);
     if (L > 0) {
       const double o2t = 0.5 / T;
       const double eT = exp(-T);
@@ -97,8 +103,10 @@ struct EriClass;
   };
 
 
-// Contracted raw quartet (canonical orientation) for one lane: loops over
-// the bra's (uniform) and the ket's primitive pairs, VRR into acc, HRR.
+// Contracted raw quartet (canonical orientation) for one lane.  The ket's
+// primitive pairs (per lane, vector loads) form the outer loop and are read
+// once; the bra's primitive pairs (wave-uniform, scalar loads) the inner
+// one.  VRR accumulates [e0|f0] over all primitive quartets, then one HRR.
 template <int LA, int LB, int LC, int LD>
 __device__ __forceinline__ void contracted_quartet(const double* __restrict__ bprim, int nbp,
                                                    const double* __restrict__ kprim, int nkp,
@@ -110,27 +118,27 @@ __device__ __forceinline__ void contracted_quartet(const double* __restrict__ bp
   double acc[C::nacc];
 #pragma unroll
   for (int k = 0; k < C::nacc; ++k) acc[k] = 0.0;
-  for (int ip = 0; ip < nbp; ++ip) {
-    const double* bp = bprim + ip * kPrimPairStride;
-    const double p = bp[0], o2p = bp[1];
-    const double Px = bp[2], Py = bp[3], Pz = bp[4];
-    const double PAx = bp[5], PAy = bp[6], PAz = bp[7];
-    const double Kab = bp[8];
-    for (int kp = 0; kp < nkp; ++kp) {
-      const double* kq = kprim + kp * kPrimPairStride;
-      const double q = kq[0], o2q = kq[1];
-      const double PQx = Px - kq[2], PQy = Py - kq[3], PQz = Pz - kq[4];
-      const double QCx = kq[5], QCy = kq[6], QCz = kq[7];
-      const double inv = 1.0 / (p + q);
+  for (int kp = 0; kp < nkp; ++kp) {
+    const double* kq = kprim + kp * kPrimPairStride;
+    const double q = kq[0], o2q = kq[1];
+    const double Qx = kq[2], Qy = kq[3], Qz = kq[4];
+    const double QCx = kq[5], QCy = kq[6], QCz = kq[7];
+    const double Kcd = kq[8];
+    for (int ip = 0; ip < nbp; ++ip) {
+      const double* bp = bprim + ip * kPrimPairStride;
+      const double p = bp[0], o2p = bp[1];
+      const double PQx = bp[2] - Qx, PQy = bp[3] - Qy, PQz = bp[4] - Qz;
+      const double rs = rsqrt_nr(p + q);
+      const double inv = rs * rs;
       const double rop = q * inv, roq = p * inv;
       const double rho = p * rop;
       const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-      const double pref = Kab * kq[8] * sqrt(inv);
+      const double pref = bp[8] * Kcd * rs;
       double F[L + 1];
       boys_f64<L>(T, boys, F);
 #pragma unroll
       for (int m = 0; m <= L; ++m) F[m] *= pref;
-      C::template vrr<double>(F, PAx, PAy, PAz, -rop * PQx, -rop * PQy, -rop * PQz, QCx, QCy, QCz,
+      C::template vrr<double>(F, bp[5], bp[6], bp[7], -rop * PQx, -rop * PQy, -rop * PQz, QCx, QCy, QCz,
                               roq * PQx, roq * PQy, roq * PQz, o2p, o2q, 0.5 * inv, rop, roq, acc);
     }
   }
@@ -169,13 +177,15 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
     if (dn * P.Q * K.Q < a.eps) active = false;
   }
   const uint64_t amask = __ballot(active);
-  if (a.counters) {
-    unsigned long long np = active ? (unsigned long long)P.nprim * (unsigned long long)K.nprim : 0ull;
+  if (a.counts) {
+    // per-wave statistics: plain stores into this task's slot (summed by
+    // reduce_counts_kernel), no same-address atomics
+    unsigned int np = active ? (unsigned int)(P.nprim * K.nprim) : 0u;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) np += __shfl_xor(np, o, 64);
     if (lane == 0) {
-      atomicAdd(&a.counters[0], (unsigned long long)__popcll(amask));
-      atomicAdd(&a.counters[1], np);
+      a.counts[2 * wid] = (unsigned int)__popcll(amask);
+      a.counts[2 * wid + 1] = np;
     }
   }
   double jab[NA * NB];
@@ -294,6 +304,171 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
     }
 }
 
+// ------------------------------------------------ J/K kernel, LDS strips
+// A workgroup owns one bra pair (a b) and streams up to t.j1 - t.j0 ket
+// pairs, one per thread per round.  Every J/K update of a quartet except
+// J_cd lands in row a or row b of G (J_ab; K_ac, K_ad in row a; K_bc, K_bd
+// in row b), so those are accumulated with LDS atomics in two row strips
+// (NA + NB) x n held for the whole ket stream and flushed to HBM once, with
+// one global atomic per touched element.  J_cd, distinct for every ket, is
+// the only per-quartet global atomic.
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(256) void jk_strip_kernel(JKArgs a) {
+  constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  extern __shared__ double strip[];  // rows of a (NA x n), then rows of b (NB x n)
+  __shared__ unsigned int wg_q, wg_p;
+  const int tid = threadIdx.x;
+  const Task t = a.tasks[blockIdx.x];
+  const PairInfo& P = a.bra[t.bra];
+  const int n = a.n, ns = a.nshell;
+  double* sa = strip;
+  double* sb = strip + NA * n;
+  for (int k = tid; k < (NA + NB) * n; k += blockDim.x) strip[k] = 0.0;
+  if (tid == 0) {
+    wg_q = 0;
+    wg_p = 0;
+  }
+  __syncthreads();
+  const double* D = a.D;
+  const double* nrm = a.norma;
+  double na[NA], nb[NB];
+#pragma unroll
+  for (int i = 0; i < NA; ++i) na[i] = nrm[P.bfa + i];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) nb[i] = nrm[P.bfb + i];
+  double dab[NA * NB];
+#pragma unroll
+  for (int i = 0; i < NA; ++i)
+#pragma unroll
+    for (int k = 0; k < NB; ++k) dab[i * NB + k] = D[(size_t)(P.bfa + i) * n + P.bfb + k] * na[i] * nb[k];
+  double jab[NA * NB];
+#pragma unroll
+  for (int k = 0; k < NA * NB; ++k) jab[k] = 0.0;
+  unsigned int myq = 0, myp = 0;
+  const double sbra = (P.sa != P.sb ? 2.0 : 1.0);
+  for (int j = t.j0 + tid; j < t.j1; j += blockDim.x) {
+    const PairInfo K = a.ket[j];
+    if (a.screen) {
+      const double* Dn = a.Dn;
+      double dn = Dn[P.sa * ns + P.sb];
+      dn = fmax(dn, Dn[K.sa * ns + K.sb]);
+      dn = fmax(dn, Dn[P.sa * ns + K.sa]);
+      dn = fmax(dn, Dn[P.sb * ns + K.sb]);
+      dn = fmax(dn, Dn[P.sa * ns + K.sb]);
+      dn = fmax(dn, Dn[P.sb * ns + K.sa]);
+      if (dn * P.Q * K.Q < a.eps) continue;
+    }
+    myq += 1;
+    myp += (unsigned int)(P.nprim * K.nprim);
+    double out[NA * NB * NC * ND];
+    contracted_quartet<LA, LB, LC, LD>(a.prim + (size_t)P.poff * kPrimPairStride, P.nprim,
+                                       a.prim + (size_t)K.poff * kPrimPairStride, K.nprim, a.boys,
+                                       P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
+    const double deg = sbra * (K.sa != K.sb ? 2.0 : 1.0) * ((a.same_class && j == t.bra) ? 1.0 : 2.0);
+    double nc[NC], nd[ND];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) nc[i] = nrm[K.bfa + i];
+#pragma unroll
+    for (int i = 0; i < ND; ++i) nd[i] = nrm[K.bfb + i];
+    double dcd[NC * ND], dac[NA * NC], dbd[NB * ND], dad[NA * ND], dbc[NB * NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i)
+#pragma unroll
+      for (int k = 0; k < ND; ++k) dcd[i * ND + k] = D[(size_t)(K.bfa + i) * n + K.bfb + k] * nc[i] * nd[k];
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+#pragma unroll
+      for (int k = 0; k < NC; ++k) dac[i * NC + k] = D[(size_t)(P.bfa + i) * n + K.bfa + k] * na[i] * nc[k];
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+      for (int k = 0; k < ND; ++k) dbd[i * ND + k] = D[(size_t)(P.bfb + i) * n + K.bfb + k] * nb[i] * nd[k];
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+#pragma unroll
+      for (int k = 0; k < ND; ++k) dad[i * ND + k] = D[(size_t)(P.bfa + i) * n + K.bfb + k] * na[i] * nd[k];
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+      for (int k = 0; k < NC; ++k) dbc[i * NC + k] = D[(size_t)(P.bfb + i) * n + K.bfa + k] * nb[i] * nc[k];
+    double jcd[NC * ND], kac[NA * NC], kbd[NB * ND], kad[NA * ND], kbc[NB * NC];
+#pragma unroll
+    for (int k = 0; k < NC * ND; ++k) jcd[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NA * NC; ++k) kac[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NB * ND; ++k) kbd[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NA * ND; ++k) kad[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NB * NC; ++k) kbc[k] = 0.0;
+#pragma unroll
+    for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+      for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+        for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+          for (int id = 0; id < ND; ++id) {
+            const double v = deg * out[((ia * NB + ib) * NC + ic) * ND + id];
+            jab[ia * NB + ib] += dcd[ic * ND + id] * v;
+            jcd[ic * ND + id] += dab[ia * NB + ib] * v;
+            kac[ia * NC + ic] += dbd[ib * ND + id] * v;
+            kbd[ib * ND + id] += dac[ia * NC + ic] * v;
+            kad[ia * ND + id] += dbc[ib * NC + ic] * v;
+            kbc[ib * NC + ic] += dad[ia * ND + id] * v;
+          }
+    const double hf = a.half_factor;
+    double* G = a.Graw;
+#pragma unroll
+    for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+      for (int id = 0; id < ND; ++id)
+        unsafeAtomicAdd(&G[(size_t)(K.bfa + ic) * n + K.bfb + id], jcd[ic * ND + id] * nc[ic] * nd[id]);
+#pragma unroll
+    for (int ia = 0; ia < NA; ++ia) {
+#pragma unroll
+      for (int ic = 0; ic < NC; ++ic) atomicAdd(&sa[ia * n + K.bfa + ic], kac[ia * NC + ic] * hf * nc[ic]);
+#pragma unroll
+      for (int id = 0; id < ND; ++id) atomicAdd(&sa[ia * n + K.bfb + id], kad[ia * ND + id] * hf * nd[id]);
+    }
+#pragma unroll
+    for (int ib = 0; ib < NB; ++ib) {
+#pragma unroll
+      for (int ic = 0; ic < NC; ++ic) atomicAdd(&sb[ib * n + K.bfa + ic], kbc[ib * NC + ic] * hf * nc[ic]);
+#pragma unroll
+      for (int id = 0; id < ND; ++id) atomicAdd(&sb[ib * n + K.bfb + id], kbd[ib * ND + id] * hf * nd[id]);
+    }
+  }
+  // J_ab of this thread into row a, column block b
+#pragma unroll
+  for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+    for (int ib = 0; ib < NB; ++ib)
+      if (jab[ia * NB + ib] != 0.0) atomicAdd(&sa[ia * n + P.bfb + ib], jab[ia * NB + ib] * nb[ib]);
+  if (a.counts) {
+    atomicAdd(&wg_q, myq);
+    atomicAdd(&wg_p, myp);
+  }
+  __syncthreads();
+  // flush: one global atomic per touched strip element (row norma applied here)
+  double* G = a.Graw;
+  for (int r = 0; r < NA; ++r)
+    for (int c = tid; c < n; c += blockDim.x) {
+      const double v = sa[r * n + c];
+      if (v != 0.0) unsafeAtomicAdd(&G[(size_t)(P.bfa + r) * n + c], v * na[r]);
+    }
+  for (int r = 0; r < NB; ++r)
+    for (int c = tid; c < n; c += blockDim.x) {
+      const double v = sb[r * n + c];
+      if (v != 0.0) unsafeAtomicAdd(&G[(size_t)(P.bfb + r) * n + c], v * nb[r]);
+    }
+  if (a.counts && tid == 0) {
+    a.counts[2 * blockIdx.x] = wg_q;
+    a.counts[2 * blockIdx.x + 1] = wg_p;
+  }
+}
+
 // ------------------------------------------------------- coupling kernel
 // Quartets (bra pair of species X | ket pair of species Y) over ALL pairs
 // s1 <= s2 of both species.  Computes the result on the bra species from the
@@ -398,6 +573,13 @@ static void launch_jk_t(const JKArgs& a, hipStream_t s) {
   if (a.ntasks <= 0) return;
   const int blocks = (a.ntasks + 3) / 4;
   hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD>), dim3(blocks), dim3(256), 0, s, a);
+}
+
+template <int LA, int LB, int LC, int LD>
+static void launch_jk_strip_t(const JKArgs& a, hipStream_t s) {
+  if (a.ntasks <= 0) return;
+  const size_t lds = (size_t)(ncart(LA) + ncart(LB)) * a.n * sizeof(double);
+  hipLaunchKernelGGL((jk_strip_kernel<LA, LB, LC, LD>), dim3(a.ntasks), dim3(256), lds, s, a);
 }
 
 template <int LA, int LB, int LC, int LD>

@@ -219,6 +219,7 @@ namespace lowdin {
 LOWDIN_ERI_CLASS(%(a)d, %(b)d, %(c)d, %(d)d)
 
 void launch_jk_%(k)s(const JKArgs& args, hipStream_t s) { launch_jk_t<%(a)d, %(b)d, %(c)d, %(d)d>(args, s); }
+void launch_jks_%(k)s(const JKArgs& args, hipStream_t s) { launch_jk_strip_t<%(a)d, %(b)d, %(c)d, %(d)d>(args, s); }
 void launch_cp_%(k)s(const CouplingArgs& args, hipStream_t s) { launch_cp_t<%(a)d, %(b)d, %(c)d, %(d)d>(args, s); }
 %(schwfn)s}  // namespace lowdin
 """
@@ -260,6 +261,7 @@ def main(outdir):
     for (la, lb, lc, ld) in classes:
         k = "%d%d%d%d" % (la, lb, lc, ld)
         lines.append("void launch_jk_%s(const JKArgs&, hipStream_t);" % k)
+        lines.append("void launch_jks_%s(const JKArgs&, hipStream_t);" % k)
         lines.append("void launch_cp_%s(const CouplingArgs&, hipStream_t);" % k)
     for (la, lb) in pcs:
         lines.append("void launch_schwarz_%d%d(PairInfo*, int, const double*, const double*, hipStream_t);" % (la, lb))
@@ -267,6 +269,11 @@ def main(outdir):
     lines.append("  switch (la * 1000 + lb * 100 + lc * 10 + ld) {")
     for (la, lb, lc, ld) in classes:
         lines.append("    case %d: launch_jk_%d%d%d%d(a, s); return true;" % (la * 1000 + lb * 100 + lc * 10 + ld, la, lb, lc, ld))
+    lines.append("    default: return false;\n  }\n}")
+    lines.append("inline bool dispatch_jk_strip(int la, int lb, int lc, int ld, const JKArgs& a, hipStream_t s) {")
+    lines.append("  switch (la * 1000 + lb * 100 + lc * 10 + ld) {")
+    for (la, lb, lc, ld) in classes:
+        lines.append("    case %d: launch_jks_%d%d%d%d(a, s); return true;" % (la * 1000 + lb * 100 + lc * 10 + ld, la, lb, lc, ld))
     lines.append("    default: return false;\n  }\n}")
     lines.append("inline bool dispatch_cp(int la, int lb, int lc, int ld, const CouplingArgs& a, hipStream_t s) {")
     lines.append("  switch (la * 1000 + lb * 100 + lc * 10 + ld) {")

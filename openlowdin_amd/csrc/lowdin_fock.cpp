@@ -146,6 +146,7 @@ struct TaskSet {
     int cbra, cket;   // pair classes of bra / ket lists
     int bra_is_a;     // coupling: bra list belongs to the first species
     size_t off, count;
+    int strip;        // intra: one workgroup per task with LDS row strips
   };
   std::vector<Launch> launches;
   DevBuf<Task> d_tasks;
@@ -180,10 +181,17 @@ struct Species {
 
   DevBuf<double> d_D, d_Dn, d_Graw, d_G, d_Dother, d_R;
   DevBuf<unsigned long long> d_cnt;
+  DevBuf<unsigned int> d_task_counts;
   lowdin_fock_stats stats{};
+  // per class launch of the last intra build: device ms, quartets, primitive quartets
+  std::vector<float> launch_ms;
+  std::vector<unsigned long long> launch_cnt;
+  std::vector<hipEvent_t> launch_ev;
+  bool time_launches = false;
 
   ~Species() {
     for (auto& kv : coupling_tasks) delete kv.second.tasks;
+    for (hipEvent_t e : launch_ev) (void)hipEventDestroy(e);
   }
 };
 
@@ -312,6 +320,13 @@ static void sort_by_cost(std::vector<Task>& t, size_t from, const PairInfo* bras
   });
 }
 
+// Largest LDS row-strip footprint a J/K workgroup may use.
+constexpr size_t kStripLdsMax = 65536;
+
+// Intra-species tasks.  Where the bra's two row strips fit in LDS, a task is
+// one workgroup: a bra pair and a run of up to `chunk` kets (chunk sized so
+// that a class launch has at least ~2048 workgroups); otherwise a task is
+// one wavefront of up to 64 kets with direct global accumulation.
 static void build_intra_tasks(Species& s, hipStream_t st) {
   TaskSet& ts = s.intra;
   ts.tasks.clear();
@@ -324,10 +339,27 @@ static void build_intra_tasks(Species& s, hipStream_t st) {
       const PairInfo* bras = ps.pairs.data() + ps.off[cx];
       const PairInfo* kets = ps.pairs.data() + ps.off[cy];
       const size_t from = ts.tasks.size();
-      for (int i = 0; i < nx; ++i) emit_tasks(ts.tasks, i, 0, cx == cy ? i + 1 : ny, kets, 0);
+      const int la = kPairClassL[cx][0], lb = kPairClassL[cx][1];
+      const bool strip = (size_t)(ncart_h(la) + ncart_h(lb)) * s.nbasis * sizeof(double) <= kStripLdsMax;
+      if (strip) {
+        long total = 0;
+        for (int i = 0; i < nx; ++i) total += (cx == cy) ? i + 1 : ny;
+        long chunk = (total + 2047) / 2048;
+        chunk = std::max(256L, std::min(8192L, ((chunk + 255) / 256) * 256));
+        for (int i = 0; i < nx; ++i) {
+          const int nk = (cx == cy) ? i + 1 : ny;
+          const int pieces = (int)((nk + chunk - 1) / chunk);
+          for (int p = 0; p < pieces; ++p) {
+            const int j0 = (int)((long)nk * p / pieces), j1 = (int)((long)nk * (p + 1) / pieces);
+            if (j1 > j0) ts.tasks.push_back(Task{i, j0, j1, 1});
+          }
+        }
+      } else {
+        for (int i = 0; i < nx; ++i) emit_tasks(ts.tasks, i, 0, cx == cy ? i + 1 : ny, kets, 0);
+      }
       sort_by_cost(ts.tasks, from, bras, kets);
-      ts.launches.push_back({kPairClassL[cx][0], kPairClassL[cx][1], kPairClassL[cy][0], kPairClassL[cy][1], cx, cy,
-                             1, from, ts.tasks.size() - from});
+      ts.launches.push_back({la, lb, kPairClassL[cy][0], kPairClassL[cy][1], cx, cy, 1, from,
+                             ts.tasks.size() - from, strip ? 1 : 0});
     }
   }
   ts.d_tasks.upload(ts.tasks.data(), ts.tasks.size(), st);
@@ -381,11 +413,16 @@ static TaskSet* coupling_taskset(Species& a, Species& b, hipStream_t st) {
       for (int i = 0; i < nbra; ++i) emit_tasks(ts->tasks, i, 0, nket, kets, 0);
       sort_by_cost(ts->tasks, from, bras, kets);
       ts->launches.push_back({kPairClassL[cbra][0], kPairClassL[cbra][1], kPairClassL[cket][0],
-                              kPairClassL[cket][1], cbra, cket, a_bra ? 1 : 0, from, ts->tasks.size() - from});
+                              kPairClassL[cket][1], cbra, cket, a_bra ? 1 : 0, from, ts->tasks.size() - from, 0});
     }
   ts->d_tasks.upload(ts->tasks.data(), ts->tasks.size(), st);
   ts->valid = true;
   return ts;
+}
+
+void set_unsupported(const char* name) {
+  g_last_error = std::string(name) + ": not provided by liblowdin_fock (outside the direct Fock path)";
+  fprintf(stderr, "lowdin_fock: %s\n", g_last_error.c_str());
 }
 
 static void fill_nan(double* out, size_t n) {
@@ -399,27 +436,30 @@ static double now_ms() {
 }
 
 // ------------------------------------------------------------ operations
-static void compute_2body(Species& s, const double* dens, double* result, double factor) {
+// Enqueue one intra-species build on stream st: D (device, n x n) -> G
+// (device, n x n).  Nothing is allocated once the species' buffers exist.
+static void enqueue_2body(Species& s, const double* dD, double* dG, double factor, hipStream_t st) {
   Context& c = ctx();
-  std::lock_guard<std::mutex> lk(c.mu);
-  const double t0 = now_ms();
-  ensure_context();
-  hipStream_t st = c.stream;
   const int n = s.nbasis, ns = (int)s.shells.size();
   const size_t nn = (size_t)n * n;
   ensure_sig(s, st);
-  s.d_D.upload(dens, nn, st);
   const bool screen = s.is_electron;
   if (screen) {
     s.d_Dn.ensure((size_t)ns * ns);
-    launch_dnorm(s.d_D.p, n, s.d_shells.p, ns, s.d_Dn.p, st);
+    launch_dnorm(dD, n, s.d_shells.p, ns, s.d_Dn.p, st);
   }
   s.d_Graw.ensure(nn);
-  s.d_G.ensure(nn);
   HIPCHK(hipMemsetAsync(s.d_Graw.p, 0, nn * sizeof(double), st));
   const size_t nl = s.intra.launches.size();
   s.d_cnt.ensure(2 * std::max<size_t>(nl, 1));
-  HIPCHK(hipMemsetAsync(s.d_cnt.p, 0, 2 * std::max<size_t>(nl, 1) * sizeof(unsigned long long), st));
+  s.d_task_counts.ensure(2 * std::max<size_t>(s.intra.tasks.size(), 1));
+  if (s.time_launches && s.launch_ev.size() < 2 * nl) {
+    while (s.launch_ev.size() < 2 * nl) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      s.launch_ev.push_back(e);
+    }
+  }
   HIPCHK(hipEventRecord(c.ev0, st));
   for (size_t k = 0; k < nl; ++k) {
     const TaskSet::Launch& L = s.intra.launches[k];
@@ -429,39 +469,70 @@ static void compute_2body(Species& s, const double* dens, double* result, double
     a.bra = s.sig.d_pairs.p + s.sig.off[L.cbra];
     a.ket = s.sig.d_pairs.p + s.sig.off[L.cket];
     a.prim = s.sig.d_prims.p;
-    a.D = s.d_D.p;
+    a.D = dD;
     a.Dn = s.d_Dn.p;
     a.norma = s.d_norma.p;
     a.boys = c.boys.p;
     a.Graw = s.d_Graw.p;
-    a.counters = s.d_cnt.p + 2 * k;
+    a.counts = s.d_task_counts.p + 2 * L.off;
     a.n = n;
     a.nshell = ns;
     a.screen = screen ? 1 : 0;
     a.same_class = L.cbra == L.cket ? 1 : 0;
     a.eps = 2.220446049250313080847e-16;  // DBL_EPSILON, Libint2Iface.h:269-271
     a.half_factor = 0.5 * factor;
-    if (!dispatch_jk(L.la, L.lb, L.lc, L.ld, a, st)) throw std::runtime_error("no J/K kernel for class");
+    if (s.time_launches) HIPCHK(hipEventRecord(s.launch_ev[2 * k], st));
+    const bool ok = L.strip ? dispatch_jk_strip(L.la, L.lb, L.lc, L.ld, a, st) : dispatch_jk(L.la, L.lb, L.lc, L.ld, a, st);
+    if (!ok) throw std::runtime_error("no J/K kernel for class");
+    if (s.time_launches) HIPCHK(hipEventRecord(s.launch_ev[2 * k + 1], st));
+  }
+  for (size_t k = 0; k < nl; ++k) {
+    const TaskSet::Launch& L = s.intra.launches[k];
+    launch_reduce_counts(s.d_task_counts.p + 2 * L.off, (int)L.count, s.d_cnt.p + 2 * k, st);
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(c.ev1, st));
-  launch_symmetrize(s.d_Graw.p, n, 1.0, s.d_G.p, st);
-  HIPCHK(hipMemcpyAsync(result, s.d_G.p, nn * sizeof(double), hipMemcpyDeviceToHost, st));
+  launch_symmetrize(s.d_Graw.p, n, 1.0, dG, st);
+  HIPCHK(hipGetLastError());
+}
+
+// After the stream has drained: fold the counters and event times into stats.
+static void collect_2body_stats(Species& s, hipStream_t st, double t0) {
+  Context& c = ctx();
+  const size_t nl = s.intra.launches.size();
   std::vector<unsigned long long> cnt(2 * std::max<size_t>(nl, 1));
   HIPCHK(hipMemcpyAsync(cnt.data(), s.d_cnt.p, cnt.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   float kms = 0.f;
   HIPCHK(hipEventElapsedTime(&kms, c.ev0, c.ev1));
   lowdin_fock_stats stt{};
+  s.launch_ms.assign(nl, 0.f);
+  s.launch_cnt = cnt;
   for (size_t k = 0; k < nl; ++k) {
     const TaskSet::Launch& L = s.intra.launches[k];
     stt.quartets += cnt[2 * k];
     stt.prim_quartets += cnt[2 * k + 1];
     stt.eris += cnt[2 * k] * (uint64_t)(ncart_h(L.la) * ncart_h(L.lb) * ncart_h(L.lc) * ncart_h(L.ld));
+    if (s.time_launches) HIPCHK(hipEventElapsedTime(&s.launch_ms[k], s.launch_ev[2 * k], s.launch_ev[2 * k + 1]));
   }
   stt.kernel_ms = kms;
   stt.total_ms = now_ms() - t0;
   s.stats = stt;
+}
+
+static void compute_2body(Species& s, const double* dens, double* result, double factor) {
+  Context& c = ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  const double t0 = now_ms();
+  ensure_context();
+  hipStream_t st = c.stream;
+  const size_t nn = (size_t)s.nbasis * s.nbasis;
+  ensure_sig(s, st);
+  s.d_D.upload(dens, nn, st);
+  s.d_G.ensure(nn);
+  enqueue_2body(s, s.d_D.p, s.d_G.p, factor, st);
+  HIPCHK(hipMemcpyAsync(result, s.d_G.p, nn * sizeof(double), hipMemcpyDeviceToHost, st));
+  collect_2body_stats(s, st, t0);
 }
 
 static void compute_coupling(Species& a, Species* b, const double* db, double* ra, const double* da, double* rb) {
@@ -662,6 +733,45 @@ void LibintInterface_compute_coupling_direct(void* lint, void* olint, double* de
 }
 
 const char* lowdin_fock_last_error(void) { return lowdin::g_last_error.c_str(); }
+
+int lowdin_fock_compute_2body_device(void* lint, const double* d_dens, double* d_result, double factor,
+                                     void* stream, int synchronize) {
+  Species* s = static_cast<Species*>(lint);
+  lowdin::Context& c = lowdin::ctx();
+  try {
+    std::lock_guard<std::mutex> lk(c.mu);
+    const double t0 = lowdin::now_ms();
+    lowdin::ensure_context();
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c.stream;
+    lowdin::enqueue_2body(*s, d_dens, d_result, factor, st);
+    if (synchronize) lowdin::collect_2body_stats(*s, st, t0);
+    lowdin::g_last_error.clear();
+    return 0;
+  } catch (const std::exception& e) {
+    lowdin::g_last_error = std::string("compute_2body_device: ") + e.what();
+    fprintf(stderr, "lowdin_fock: %s\n", lowdin::g_last_error.c_str());
+    return -1;
+  }
+}
+
+int lowdin_fock_time_launches(void* lint, int enable) {
+  static_cast<Species*>(lint)->time_launches = enable != 0;
+  return 0;
+}
+
+int lowdin_fock_launch_profile(void* lint, int cap, int* classes, float* ms, uint64_t* quartets,
+                               uint64_t* prim_quartets) {
+  Species* s = static_cast<Species*>(lint);
+  const int nl = (int)s->intra.launches.size();
+  for (int k = 0; k < nl && k < cap; ++k) {
+    const lowdin::TaskSet::Launch& L = s->intra.launches[k];
+    classes[k] = L.la * 1000 + L.lb * 100 + L.lc * 10 + L.ld;
+    ms[k] = k < (int)s->launch_ms.size() ? s->launch_ms[k] : 0.f;
+    quartets[k] = 2 * k < (int)s->launch_cnt.size() ? s->launch_cnt[2 * k] : 0;
+    prim_quartets[k] = 2 * k + 1 < (int)s->launch_cnt.size() ? s->launch_cnt[2 * k + 1] : 0;
+  }
+  return nl;
+}
 
 int lowdin_fock_get_stats(void* lint, lowdin_fock_stats* out) {
   if (!lint || !out) return -1;

@@ -19,7 +19,7 @@ struct JKArgs {
   const double* norma;
   const double* boys;
   double* Graw;            // n x n raw accumulator (symmetrised afterwards)
-  unsigned long long* counters;  // [0] computed quartets, [1] primitive quartets (optional)
+  unsigned int* counts;     // per task: computed quartets, primitive quartets (optional)
   int n;
   int nshell;
   int screen;              // 1: Dnorm*Q*Q < eps screen (electrons)
@@ -96,5 +96,6 @@ void launch_pair_norms(const ShellInfo* sh, const double* pexp, const double* pc
                        hipStream_t s);
 void launch_dnorm(const double* D, int n, const ShellInfo* sh, int nsh, double* Dn, hipStream_t s);
 void launch_symmetrize(const double* Graw, int n, double scale, double* G, hipStream_t s);
+void launch_reduce_counts(const unsigned int* counts, int ntasks, unsigned long long* out, hipStream_t s);
 
 }  // namespace lowdin

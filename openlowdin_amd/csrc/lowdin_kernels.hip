@@ -56,7 +56,36 @@ __global__ void symmetrize_kernel(const double* Graw, int n, double scale, doubl
   G[(size_t)i * n + k] = scale * 0.25 * (Graw[(size_t)i * n + k] + Graw[(size_t)k * n + i]);
 }
 
+// Sum the per-task (quartets, primitive quartets) pairs of one launch.
+__global__ __launch_bounds__(1024) void reduce_counts_kernel(const unsigned int* counts, int ntasks,
+                                                             unsigned long long* out) {
+  __shared__ unsigned long long sq[1024], sp[1024];
+  unsigned long long q = 0, p = 0;
+  for (int i = threadIdx.x; i < ntasks; i += blockDim.x) {
+    q += counts[2 * i];
+    p += counts[2 * i + 1];
+  }
+  sq[threadIdx.x] = q;
+  sp[threadIdx.x] = p;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sq[threadIdx.x] += sq[threadIdx.x + w];
+      sp[threadIdx.x] += sp[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = sq[0];
+    out[1] = sp[0];
+  }
+}
+
 // ------------------------------------------------------------ launchers
+void launch_reduce_counts(const unsigned int* counts, int ntasks, unsigned long long* out, hipStream_t s) {
+  hipLaunchKernelGGL(reduce_counts_kernel, dim3(1), dim3(1024), 0, s, counts, ntasks, out);
+}
+
 void launch_pair_norms(const ShellInfo* sh, const double* pexp, const double* pcoef, int nsh, double* norms,
                        hipStream_t s) {
   const long npair = (long)nsh * (nsh + 1) / 2;

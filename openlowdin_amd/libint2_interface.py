@@ -68,6 +68,12 @@ def lib():
         L.lowdin_fock_schwarz.restype = C.c_int
         L.lowdin_fock_coupling_pair.argtypes = [vp, vp, dp, dp, dp, dp]
         L.lowdin_fock_coupling_pair.restype = C.c_int
+        L.lowdin_fock_compute_2body_device.argtypes = [vp, vp, vp, C.c_double, vp, C.c_int]
+        L.lowdin_fock_compute_2body_device.restype = C.c_int
+        L.lowdin_fock_time_launches.argtypes = [vp, C.c_int]
+        L.lowdin_fock_launch_profile.argtypes = [vp, C.c_int, ip, C.POINTER(C.c_float),
+                                                 C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        L.lowdin_fock_launch_profile.restype = C.c_int
         _LIB = L
     return _LIB
 
@@ -157,6 +163,33 @@ class Libint2Interface:
             _check("coupling_pair")
             raise LowdinFockError("coupling_pair failed")
         return Ra, Rb
+
+    def compute_2body_device(self, d_dens: int, d_result: int, factor: float, stream: int = 0,
+                             synchronize: bool = True):
+        """Device-resident build: d_dens / d_result are device addresses of
+        n x n FP64 buffers; the work is enqueued on ``stream`` (hipStream_t
+        address, 0 = the library's stream)."""
+        rc = lib().lowdin_fock_compute_2body_device(self.this, d_dens, d_result, factor, stream or None,
+                                                     1 if synchronize else 0)
+        if rc != 0:
+            _check("compute_2body_device")
+            raise LowdinFockError("compute_2body_device failed")
+
+    def time_launches(self, enable: bool = True):
+        lib().lowdin_fock_time_launches(self.this, 1 if enable else 0)
+
+    def launch_profile(self):
+        """[(class code, ms, quartets, primitive quartets)] of the last intra build."""
+        cap = 64
+        cls = np.zeros(cap, dtype=np.int32)
+        ms = np.zeros(cap, dtype=np.float32)
+        q = np.zeros(cap, dtype=np.uint64)
+        pq = np.zeros(cap, dtype=np.uint64)
+        n = lib().lowdin_fock_launch_profile(self.this, cap, cls.ctypes.data_as(C.POINTER(C.c_int)),
+                                             ms.ctypes.data_as(C.POINTER(C.c_float)),
+                                             q.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                             pq.ctypes.data_as(C.POINTER(C.c_uint64)))
+        return [("%04d" % cls[k], float(ms[k]), int(q[k]), int(pq[k])) for k in range(min(n, cap))]
 
     def stats(self) -> dict:
         s = Stats()

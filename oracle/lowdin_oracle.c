@@ -253,8 +253,9 @@ static void shell_xyz(const orc_basis *bs, int s, double X[3]) {
 
 /* Raw contracted ERIs (libint-normalised primitives, raw coefficients, no
  * norma) for shells (s1 s2 | s3 s4); s1, s2 from basis b12; s3, s4 from b34.
- * Primitive quartets whose prefactor bound |K_ab K_cd| 2 pi^2.5/(pq sqrt(p+q))
- * falls below prim_eps are skipped (prim_eps = 0 keeps all). */
+ * With prim_eps > 0, primitive pairs whose Gaussian-product factor
+ * exp(-ab/(a+b) |AB|^2) is below prim_eps are skipped (the same pair-level
+ * truncation the GPU build applies at 1e-30); prim_eps = 0 keeps all. */
 void orc_eri_shell(const orc_basis *b12, int s1, int s2, const orc_basis *b34, int s3,
                    int s4, double prim_eps, double *out) {
   const int l1 = b12->l[s1], l2 = b12->l[s2], l3 = b34->l[s3], l4 = b34->l[s4];
@@ -268,23 +269,21 @@ void orc_eri_shell(const orc_basis *b12, int s1, int s2, const orc_basis *b34, i
     ab2 += (A[k] - B[k]) * (A[k] - B[k]);
     cd2 += (C[k] - D[k]) * (C[k] - D[k]);
   }
+  const double cut = prim_eps > 0.0 ? -log(prim_eps) : INFINITY;
   for (int i = 0; i < b12->nprim[s1]; ++i) {
     const double a = b12->exps[b12->poff[s1] + i];
     const double ca = b12->coefs[b12->poff[s1] + i] * orc_prim_norm(a, l1);
     for (int j = 0; j < b12->nprim[s2]; ++j) {
       const double b = b12->exps[b12->poff[s2] + j];
+      if (a * b / (a + b) * ab2 > cut) continue;
       const double cb = b12->coefs[b12->poff[s2] + j] * orc_prim_norm(b, l2);
-      const double p = a + b, kab = ca * cb * exp(-a * b / p * ab2);
       for (int k = 0; k < b34->nprim[s3]; ++k) {
         const double c = b34->exps[b34->poff[s3] + k];
         const double cc = b34->coefs[b34->poff[s3] + k] * orc_prim_norm(c, l3);
         for (int l = 0; l < b34->nprim[s4]; ++l) {
           const double d = b34->exps[b34->poff[s4] + l];
+          if (c * d / (c + d) * cd2 > cut) continue;
           const double cd = b34->coefs[b34->poff[s4] + l] * orc_prim_norm(d, l4);
-          const double q = c + d, kcd = cc * cd * exp(-c * d / q * cd2);
-          if (prim_eps > 0.0 &&
-              fabs(kab * kcd) * 2.0 * pow(ORC_PI, 2.5) / (p * q * sqrt(p + q)) < prim_eps)
-            continue;
           orc_prim_eri(l1, l2, l3, l4, A, B, C, D, a, b, c, d, prim);
           const double w = ca * cb * cc * cd;
           for (int m = 0; m < n; ++m) out[m] += w * prim[m];
@@ -401,11 +400,12 @@ void orc_schwarz(const orc_basis *bs, double *Q) {
  * D[i*n + j] (symmetric).  pairs: output of orc_shellpair_list.  Q: Schwarz
  * matrix (nshell^2).  Returns the reference's num_ints_computed.
  * counters (optional, 2 longs): [0] computed shell quartets, [1] primitive
- * quartets before primitive screening. */
+ * quartets before primitive screening.  s1_stride > 1 restricts the outer
+ * shell loop to s1 = 0 mod s1_stride (a bounded sample for CPU timing). */
 long orc_compute_2body_direct(const orc_basis *bs, const int *pairs_s1, const int *pairs_s2,
                               int npairs, const double *Q, const double *Dmat, double factor,
                               int screen, double precision, double prim_eps, int nthreads,
-                              double *G, long *counters) {
+                              int s1_stride, double *G, long *counters) {
   const int ns = bs->nshell, n = orc_nbasis(bs);
   int *off = (int *)malloc(sizeof(int) * ns);
   shell_offsets(bs, off);
@@ -451,6 +451,7 @@ long orc_compute_2body_direct(const orc_basis *bs, const int *pairs_s1, const in
     double *buf = (double *)malloc(sizeof(double) * ORC_MAXNC * ORC_MAXNC * ORC_MAXNC * ORC_MAXNC);
     long s1234 = 0;
     for (int s1 = 0; s1 < ns; ++s1) {
+      if (s1_stride > 1 && s1 % s1_stride != 0) continue;
       const int n1 = ncart(bs->l[s1]), bf1f = off[s1];
       for (int k2 = start[s1]; k2 < start[s1 + 1]; ++k2) {
         const int s2 = plist[k2], n2 = ncart(bs->l[s2]), bf2f = off[s2];

@@ -51,7 +51,7 @@ def lib():
         L.orc_shellpair_list.restype = C.c_int
         L.orc_schwarz.argtypes = [C.c_void_p, dp]
         L.orc_compute_2body_direct.argtypes = [C.c_void_p, ip, ip, C.c_int, dp, dp, C.c_double,
-                                               C.c_int, C.c_double, C.c_double, C.c_int, dp, lp]
+                                               C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, dp, lp]
         L.orc_compute_2body_direct.restype = C.c_long
         L.orc_compute_coupling_direct.argtypes = [C.c_void_p, C.c_void_p, dp, C.c_int, C.c_double,
                                                   C.c_int, dp, lp]
@@ -133,7 +133,7 @@ def schwarz(b: Basis) -> np.ndarray:
 
 def compute_2body_direct(b: Basis, D: np.ndarray, factor: float, pairs=None, Q=None,
                          precision: float = DBL_EPSILON, prim_eps: float = 0.0,
-                         nthreads: int = 0, return_counts: bool = False):
+                         nthreads: int = 0, return_counts: bool = False, s1_stride: int = 1):
     """LibintInterface::compute_2body_direct (Libint2Iface.cpp:418-618) with
     the C wrapper's per-call pair list and Schwarz matrix (3505, 3520)."""
     if pairs is None:
@@ -147,7 +147,7 @@ def compute_2body_direct(b: Basis, D: np.ndarray, factor: float, pairs=None, Q=N
     screen = 1 if b.species.is_electron else 0
     nints = lib().orc_compute_2body_direct(b.ptr, s1, s2, len(s1), np.ascontiguousarray(Q, dtype=np.float64).reshape(-1),
                                            np.ascontiguousarray(D, dtype=np.float64).reshape(-1), float(factor),
-                                           screen, precision, prim_eps, nthreads, G, counters)
+                                           screen, precision, prim_eps, nthreads, s1_stride, G, counters)
     G = G.reshape(b.n, b.n)
     if return_counts:
         return G, {"eris": int(nints), "quartets": int(counters[0]), "prim_quartets": int(counters[1])}

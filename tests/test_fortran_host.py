@@ -1,0 +1,76 @@
+"""The Fortran host layer (ISO_C_BINDING, built with amdflang) drives the
+GPU build exactly as openLOWDIN's WaveFunction direct branches do; its raw
+library results are checked against the oracle and its scaled species
+matrices against the reference's caller semantics (q^2 scaling, the
+one-particle skip, q_i q_j scaling and upper-to-lower symmetrisation)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from openlowdin_amd import system as S
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(REPO, "openlowdin_amd", "fortran", "fock_host_driver")
+
+
+def _write_job(d, system, dens):
+    with open(os.path.join(d, "job.txt"), "w") as fh:
+        fh.write("%d\n" % len(system.species))
+        for k, sp in enumerate(system.species):
+            fh.write("%d %d %.17g %.17g %.17g %d %d\n" % (k + 1, int(sp.is_electron), sp.charge, sp.kappa, sp.eta,
+                                                         sp.n_particles, len(sp.shells)))
+            for sh in sp.shells:
+                fh.write("%d %d %.17g %.17g %.17g\n" % (sh.l, sh.nprim, *sh.origin))
+                fh.write(" ".join("%.17g" % x for x in sh.exps) + "\n")
+                fh.write(" ".join("%.17g" % x for x in sh.coefs) + "\n")
+    for k, D in enumerate(dens):
+        np.asfortranarray(D).ravel(order="F").tofile(os.path.join(d, "dens_%d.bin" % (k + 1)))
+
+
+def _read(d, name, n):
+    return np.fromfile(os.path.join(d, name), dtype=np.float64).reshape((n, n), order="F")
+
+
+def test_driver_is_built():
+    assert os.path.exists(DRIVER), "run __graft_entry__.build() (amdflang driver)"
+
+
+@pytest.mark.gpu
+def test_fortran_driver_fhf_anion(tmp_path):
+    system = S.fhf_anion("6-31G")
+    e, p = system.species
+    rng = np.random.default_rng(23)
+    Ce = rng.normal(size=(e.nbasis, 10)) / np.sqrt(e.nbasis)
+    De = 2.0 * Ce @ Ce.T
+    cp = rng.normal(size=p.nbasis) / np.sqrt(p.nbasis)
+    Dp = np.outer(cp, cp)
+    _write_job(str(tmp_path), system, [De, Dp])
+    print("\n[case] fortran driver: FHF- 6-31G + proton", flush=True)
+    r = subprocess.run([DRIVER, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    be, bp = O.Basis(e), O.Basis(p)
+    # raw results = the oracle (reference semantics) to 1e-10
+    Ge = _read(str(tmp_path), "g_1.bin", e.nbasis)
+    assert np.max(np.abs(Ge - O.compute_2body_direct(be, De, -0.5))) < 1e-10
+    Gp = _read(str(tmp_path), "g_2.bin", p.nbasis)
+    assert np.max(np.abs(Gp - O.compute_2body_direct(bp, Dp, -1.0))) < 1e-10
+    c12 = _read(str(tmp_path), "c_1_2.bin", e.nbasis)
+    ref12 = O.compute_coupling_direct(be, bp, 1, 2, Dp)
+    assert np.max(np.abs(c12.ravel(order="F") - ref12.ravel())) < 1e-10
+    c21 = _read(str(tmp_path), "c_2_1.bin", p.nbasis)
+    ref21 = O.compute_coupling_direct(bp, be, 2, 1, De)
+    assert np.max(np.abs(c21.ravel(order="F") - ref21.ravel())) < 1e-10
+    # caller semantics: electrons have 20 particles (G scaled by q^2 = 1),
+    # the single proton's G is not built; coupling scaled by q_e q_p = -1
+    two1 = _read(str(tmp_path), "two_1.bin", e.nbasis)
+    assert np.max(np.abs(two1 - Ge)) == 0.0
+    two2 = _read(str(tmp_path), "two_2.bin", p.nbasis)
+    assert np.max(np.abs(two2)) == 0.0
+    cpl1 = _read(str(tmp_path), "coupling_1.bin", e.nbasis)
+    up = np.triu(c12)
+    expect = -(up + np.triu(up, 1).T)
+    assert np.max(np.abs(cpl1 - expect)) < 1e-14
