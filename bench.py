@@ -60,7 +60,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-stride", type=int, default=16)
+    ap.add_argument("--cpu-stride", type=int, default=4)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a 256 MiB device-to-device copy (profiler byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -70,6 +72,13 @@ def main():
     from openlowdin_amd import hip as H
     from openlowdin_amd import libint2_interface as LI
     from openlowdin_amd import system as S
+
+    if args.copy_calibration:
+        src, dst = H.DeviceArray((1 << 25,)), H.DeviceArray((1 << 25,))   # 256 MiB each
+        dst.copy_from(src)
+        H.hip().hipDeviceSynchronize()
+        src.free()
+        dst.free()
 
     sp = S.alkane(20, "cc-pVDZ").species[0]
     n = sp.nbasis
@@ -85,38 +94,57 @@ def main():
     print("[bench] warm: %d quartets, %d ERIs, %.2f ms kernels" % (st["quartets"], st["eris"], st["kernel_ms"]),
           file=sys.stderr, flush=True)
 
-    # timed region: K builds back to back on one stream, inputs resident
-    h.time_launches(True)
-    per_class = {}
+    # timed region: K builds back to back, inputs resident in HBM
     e0, e1 = H.Event(), H.Event()
     stream.synchronize()
     t0 = time.perf_counter()
     e0.record(stream)
     for k in range(args.steps):
-        h.compute_2body_device(dD.ptr, dG.ptr, -0.5, stream.handle, synchronize=True)
-        for code, ms, q, p in h.launch_profile():
-            acc = per_class.setdefault(code, [0.0, q, p])
-            acc[0] += ms
+        h.compute_2body_device(dD.ptr, dG.ptr, -0.5, stream.handle, synchronize=False)
     e1.record(stream)
     e1.synchronize()
     t1 = time.perf_counter()
-    h.time_launches(False)
+    h.compute_2body_device(dD.ptr, dG.ptr, -0.5, stream.handle, synchronize=True)
     st = h.stats()
     wall = t1 - t0
     ev_ms = e0.elapsed_ms(e1)
     ms_per_step = wall * 1e3 / args.steps
     builds_per_s = args.steps / wall
 
+    # per-class timing pass: the class launches serialised on one stream, so
+    # each kernel's HIP-event duration is its own
+    h.time_launches(True)
+    h.serialize_launches(True)
+    per_class = {}
+    nprof = max(3, min(args.steps, 5))
+    for k in range(nprof):
+        h.compute_2body_device(dD.ptr, dG.ptr, -0.5, stream.handle, synchronize=True)
+        for code, ms, q, p in h.launch_profile():
+            acc = per_class.setdefault(code, [0.0, q, p])
+            acc[0] += ms
+    h.time_launches(False)
+    h.serialize_launches(False)
+
     # roofline of the dominant class kernel
     code, (tot_ms, q, p) = max(per_class.items(), key=lambda kv: kv[1][0])
-    avg_ms = tot_ms / args.steps
+    avg_ms = tot_ms / nprof
     fl = FL.class_flops(code, q, p)
     achieved = fl / (avg_ms * 1e-3) / 1e12
     total_flops = sum(FL.class_flops(c, v[1], v[2]) for c, v in per_class.items())
-    classes = {c: {"ms": round(v[0] / args.steps, 4), "quartets": v[1], "prim_quartets": v[2],
-                   "tflops": round(FL.class_flops(c, v[1], v[2]) / (v[0] / args.steps * 1e-3) / 1e12, 3)
+    classes = {c: {"ms": round(v[0] / nprof, 4), "quartets": v[1], "prim_quartets": v[2],
+                   "tflops": round(FL.class_flops(c, v[1], v[2]) / (v[0] / nprof * 1e-3) / 1e12, 3)
                    if v[0] > 0 else None}
                for c, v in sorted(per_class.items(), key=lambda kv: -kv[1][0])}
+
+    # HBM traffic of the dominant kernel: measured offline with rocprofv3
+    # FETCH_SIZE / WRITE_SIZE (copy-calibrated) and committed under profiles/
+    traffic = None
+    tpath = os.path.join(HERE, "profiles", "jk_c20h42_fp64_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tk = json.load(fh)["kernels"].get("jk_strip_kernel<%s>" % ", ".join(code))
+        if tk:
+            traffic = tk["fetch_bytes"] + tk["write_bytes"]
 
     out = {
         "metric": METRIC,
@@ -140,9 +168,13 @@ def main():
         "prim_quartets_per_build": st["prim_quartets"],
         "event_ms_per_step": round(ev_ms / args.steps, 4),
         "build_tflops_fp64": round(total_flops * builds_per_s / 1e12, 3),
-        "roofline": {"bound": "valu", "kernel": "jk_kernel<%s>" % ",".join(code), "achieved": round(achieved, 3),
-                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                     "traffic": None, "flops_per_launch": fl, "avg_launch_ms": round(avg_ms, 4)},
+        "roofline": {"bound": "valu", "kernel": "jk_strip_kernel<%s>" % ",".join(code),
+                     "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic, "traffic_unit": "bytes/launch",
+                     "traffic_source": "profiles/jk_c20h42_fp64_traffic.md" if traffic else None, "flops_per_launch": fl,
+                     "avg_launch_ms": round(avg_ms, 4),
+                     "timing": "HIP events around the kernel, class launches serialised (%d builds)" % nprof},
+        "serialized_ms_per_build": round(sum(v[0] for v in per_class.values()) / nprof, 4),
         "classes": classes,
     }
     if not args.no_cpu_baseline:

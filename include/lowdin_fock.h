@@ -115,6 +115,10 @@ int lowdin_fock_compute_2body_device(void* lint, const double* d_dens, double* d
 /* Record HIP events around every class launch of the intra build. */
 int lowdin_fock_time_launches(void* lint, int enable);
 
+/* Run the class launches of the intra build one after another on one
+ * stream instead of overlapping them on several (for per-class timing). */
+int lowdin_fock_serialize_launches(void* lint, int enable);
+
 /* Per class launch of the last intra build: class code 1000*la+100*lb+10*lc+ld,
  * device ms (when timing is on), shell quartets and primitive quartets.
  * Returns the number of launches. */

@@ -67,9 +67,13 @@ struct DevBuf {
 };
 
 // ------------------------------------------------------------ process state
+constexpr int kSideStreams = 4;  // class launches overlap on these (default priority)
+
 struct Context {
   bool ready = false;
   hipStream_t stream = nullptr;
+  hipStream_t side[kSideStreams] = {};
+  hipEvent_t fork = nullptr, join[kSideStreams] = {};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   DevBuf<double> boys;
   std::mutex mu;  // one process stream: calls are serialised
@@ -110,6 +114,11 @@ static void ensure_context() {
   HIPCHK(hipGetDeviceCount(&ndev));
   if (ndev < 1) throw std::runtime_error("no HIP device visible");
   HIPCHK(hipStreamCreate(&c.stream));  // default priority
+  for (int k = 0; k < kSideStreams; ++k) {
+    HIPCHK(hipStreamCreate(&c.side[k]));
+    HIPCHK(hipEventCreateWithFlags(&c.join[k], hipEventDisableTiming));
+  }
+  HIPCHK(hipEventCreateWithFlags(&c.fork, hipEventDisableTiming));
   HIPCHK(hipEventCreate(&c.ev0));
   HIPCHK(hipEventCreate(&c.ev1));
   std::vector<double> tab = make_boys_table();
@@ -149,6 +158,7 @@ struct TaskSet {
     int strip;        // intra: one workgroup per task with LDS row strips
   };
   std::vector<Launch> launches;
+  std::vector<size_t> order;   // launch order: estimated cost, largest first
   DevBuf<Task> d_tasks;
 };
 
@@ -188,6 +198,7 @@ struct Species {
   std::vector<unsigned long long> launch_cnt;
   std::vector<hipEvent_t> launch_ev;
   bool time_launches = false;
+  bool serialize = false;   // all class launches on one side stream (clean per-class timing)
 
   ~Species() {
     for (auto& kv : coupling_tasks) delete kv.second.tasks;
@@ -362,6 +373,21 @@ static void build_intra_tasks(Species& s, hipStream_t st) {
                              ts.tasks.size() - from, strip ? 1 : 0});
     }
   }
+  // estimated cost of each class launch: primitive quartets x class weight
+  std::vector<double> cost(ts.launches.size(), 0.0);
+  for (size_t k = 0; k < ts.launches.size(); ++k) {
+    const TaskSet::Launch& L = ts.launches[k];
+    const PairInfo* bras = ps.pairs.data() + ps.off[L.cbra];
+    const PairInfo* kets = ps.pairs.data() + ps.off[L.cket];
+    const double w = 1.0 + (L.la + L.lb + L.lc + L.ld) * (L.la + L.lb + L.lc + L.ld);
+    for (size_t t = L.off; t < L.off + L.count; ++t) {
+      const Task& tk = ts.tasks[t];
+      cost[k] += w * (double)bras[tk.bra].nprim * kets[tk.j0].nprim * (tk.j1 - tk.j0);
+    }
+  }
+  ts.order.resize(ts.launches.size());
+  for (size_t k = 0; k < ts.order.size(); ++k) ts.order[k] = k;
+  std::stable_sort(ts.order.begin(), ts.order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
   ts.d_tasks.upload(ts.tasks.data(), ts.tasks.size(), st);
   ts.valid = true;
 }
@@ -460,9 +486,15 @@ static void enqueue_2body(Species& s, const double* dD, double* dG, double facto
       s.launch_ev.push_back(e);
     }
   }
+  // Class launches fan out over the side streams, largest first, so that
+  // small latency-bound classes overlap the large ones.
   HIPCHK(hipEventRecord(c.ev0, st));
-  for (size_t k = 0; k < nl; ++k) {
+  HIPCHK(hipEventRecord(c.fork, st));
+  for (int q = 0; q < kSideStreams; ++q) HIPCHK(hipStreamWaitEvent(c.side[q], c.fork, 0));
+  for (size_t r = 0; r < nl; ++r) {
+    const size_t k = s.intra.order[r];
     const TaskSet::Launch& L = s.intra.launches[k];
+    hipStream_t ls = c.side[s.serialize ? 0 : r % kSideStreams];
     JKArgs a{};
     a.tasks = s.intra.d_tasks.p + L.off;
     a.ntasks = (int)L.count;
@@ -481,10 +513,14 @@ static void enqueue_2body(Species& s, const double* dD, double* dG, double facto
     a.same_class = L.cbra == L.cket ? 1 : 0;
     a.eps = 2.220446049250313080847e-16;  // DBL_EPSILON, Libint2Iface.h:269-271
     a.half_factor = 0.5 * factor;
-    if (s.time_launches) HIPCHK(hipEventRecord(s.launch_ev[2 * k], st));
-    const bool ok = L.strip ? dispatch_jk_strip(L.la, L.lb, L.lc, L.ld, a, st) : dispatch_jk(L.la, L.lb, L.lc, L.ld, a, st);
+    if (s.time_launches) HIPCHK(hipEventRecord(s.launch_ev[2 * k], ls));
+    const bool ok = L.strip ? dispatch_jk_strip(L.la, L.lb, L.lc, L.ld, a, ls) : dispatch_jk(L.la, L.lb, L.lc, L.ld, a, ls);
     if (!ok) throw std::runtime_error("no J/K kernel for class");
-    if (s.time_launches) HIPCHK(hipEventRecord(s.launch_ev[2 * k + 1], st));
+    if (s.time_launches) HIPCHK(hipEventRecord(s.launch_ev[2 * k + 1], ls));
+  }
+  for (int q = 0; q < kSideStreams; ++q) {
+    HIPCHK(hipEventRecord(c.join[q], c.side[q]));
+    HIPCHK(hipStreamWaitEvent(st, c.join[q], 0));
   }
   for (size_t k = 0; k < nl; ++k) {
     const TaskSet::Launch& L = s.intra.launches[k];
@@ -756,6 +792,11 @@ int lowdin_fock_compute_2body_device(void* lint, const double* d_dens, double* d
 
 int lowdin_fock_time_launches(void* lint, int enable) {
   static_cast<Species*>(lint)->time_launches = enable != 0;
+  return 0;
+}
+
+int lowdin_fock_serialize_launches(void* lint, int enable) {
+  static_cast<Species*>(lint)->serialize = enable != 0;
   return 0;
 }
 

@@ -95,6 +95,10 @@ class DeviceArray:
             check(hip().hipMemcpyAsync(self.ptr, a.ctypes.data, self.nbytes, H2D, stream.handle), "H2D")
             stream.synchronize()
 
+    def copy_from(self, other: "DeviceArray"):
+        """Device-to-device copy of min(nbytes) bytes."""
+        check(hip().hipMemcpy(self.ptr, other.ptr, min(self.nbytes, other.nbytes), D2D), "hipMemcpy D2D")
+
     def download(self, order="C") -> np.ndarray:
         out = np.empty(self.shape, dtype=self.dtype)
         check(hip().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, D2H), "hipMemcpy D2H")

@@ -71,6 +71,7 @@ def lib():
         L.lowdin_fock_compute_2body_device.argtypes = [vp, vp, vp, C.c_double, vp, C.c_int]
         L.lowdin_fock_compute_2body_device.restype = C.c_int
         L.lowdin_fock_time_launches.argtypes = [vp, C.c_int]
+        L.lowdin_fock_serialize_launches.argtypes = [vp, C.c_int]
         L.lowdin_fock_launch_profile.argtypes = [vp, C.c_int, ip, C.POINTER(C.c_float),
                                                  C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.lowdin_fock_launch_profile.restype = C.c_int
@@ -177,6 +178,9 @@ class Libint2Interface:
 
     def time_launches(self, enable: bool = True):
         lib().lowdin_fock_time_launches(self.this, 1 if enable else 0)
+
+    def serialize_launches(self, enable: bool = True):
+        lib().lowdin_fock_serialize_launches(self.this, 1 if enable else 0)
 
     def launch_profile(self):
         """[(class code, ms, quartets, primitive quartets)] of the last intra build."""
