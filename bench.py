@@ -54,6 +54,114 @@ def cpu_baseline(sp, D, stride, eris_full):
                       "by ERI count" % (stride, cnt["eris"], eris_full, dt)}
 
 
+def time_device_builds(h, dD, dG, factor, stream, steps, H):
+    """Seconds per device-resident build, K builds back to back."""
+    h.compute_2body_device(dD.ptr, dG.ptr, factor, stream.handle, synchronize=True)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        h.compute_2body_device(dD.ptr, dG.ptr, factor, stream.handle, synchronize=False)
+    stream.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def mixed_sweep(system, steps, taus, H, LI, WF, FL, stream):
+    """C20H42 SCF in FP64 and in mixed precision for each tau: total-energy
+    error of the converged SCF, Fock (G) error and build rate at the
+    converged FP64 density."""
+    sp = system.species[0]
+    n = sp.nbasis
+    hcore, overlap = WF.one_body_matrices(system, {})
+    h64 = LI.Libint2Interface(sp, 1)
+    t0 = time.perf_counter()
+    E64, wfs = WF.run_scf(system, hcore=hcore, overlap=overlap, objs={1: h64}, e_tol=1e-10, d_tol=1e-7)
+    t_scf = time.perf_counter() - t0
+    D = wfs[0].density
+    G64 = h64.compute_2body_intraspecies_direct(D, sp.exchange_factor)
+    dD = H.DeviceArray.from_host(np.asfortranarray(D).ravel(order="F"))
+    dG = H.DeviceArray((n * n,))
+    s64 = time_device_builds(h64, dD, dG, sp.exchange_factor, stream, steps, H)
+    st64 = h64.stats()
+    rows = []
+    for tau in taus:
+        hm = LI.Libint2Interface(sp, 1)
+        hm.set_precision("mixed", tau)
+        Em, _ = WF.run_scf(system, hcore=hcore, overlap=overlap, objs={1: hm}, e_tol=1e-10, d_tol=1e-7)
+        Gm = hm.compute_2body_intraspecies_direct(D, sp.exchange_factor)
+        st = hm.stats()
+        sm = time_device_builds(hm, dD, dG, sp.exchange_factor, stream, steps, H)
+        rows.append({"tau": tau, "builds_per_s": round(1.0 / sm, 3), "speedup_vs_fp64": round(s64 / sm, 3),
+                     "dE_scf": Em - E64, "max_abs_dF": float(np.max(np.abs(Gm - G64))),
+                     "dE_fixed_density": 0.5 * float(np.sum(D * (Gm - G64))),
+                     "fp32_quartet_share": round(st["quartets_fp32"] / max(st["quartets"], 1), 4),
+                     "fp32_eri_share": round(st["eris_fp32"] / max(st["eris"], 1), 4)})
+        print("[bench] mixed tau %.0e: %.2f builds/s (x%.2f), dE %.2e, max|dF| %.2e" %
+              (tau, 1.0 / sm, s64 / sm, Em - E64, rows[-1]["max_abs_dF"]), file=sys.stderr, flush=True)
+    ok = [r for r in rows if abs(r["dE_scf"]) < 1e-6]
+    best = max(ok, key=lambda r: r["builds_per_s"]) if ok else None
+    return {"E_fp64": E64, "scf_seconds_fp64": round(t_scf, 3), "scf_iterations_note": "host DIIS + eigh, GPU builds",
+            "fp64_builds_per_s": round(1.0 / s64, 3), "eris_per_build": st64["eris"], "sweep": rows,
+            "best_tau_within_1e-6": best["tau"] if best else None,
+            "best_mixed_builds_per_s": best["builds_per_s"] if best else None,
+            "best_speedup_vs_fp64": best["speedup_vs_fp64"] if best else None}
+
+
+def batch_bench(steps, H, B, S, mixed_tau, stream):
+    """The 256 water-cluster batch: one class launch per quartet class over
+    all molecules, replayed as a HIP graph after autotuning; against the same
+    256 builds one molecule per call."""
+    from openlowdin_amd import libint2_interface as LI
+    systems = S.water_cluster_batch(256)
+    sps = [s.species[0] for s in systems]
+    rng = np.random.default_rng(23)
+    Ds = [synthetic_density(sp.nbasis, sp.n_occupied, seed=int(rng.integers(1 << 30))) for sp in sps]
+    out = {"molecules": len(sps), "n_range": [min(sp.nbasis for sp in sps), max(sp.nbasis for sp in sps)],
+           "density": "2 C C^T of random orbitals per molecule"}
+    for mode in ("fp64", "mixed"):
+        fb = B.FockBatch.from_species(sps)
+        if mode == "mixed":
+            for h in fb.handles:
+                h.set_precision("mixed", mixed_tau)
+        dD = H.DeviceArray.from_host(fb.pack(Ds))
+        dG = H.DeviceArray((fb.packed_size,))
+        fb.compute_device(dD.ptr, dG.ptr, stream.handle)
+        t0 = time.perf_counter()
+        fb.autotune(dD.ptr, dG.ptr, reps=1, stream=stream.handle)
+        t_tune = time.perf_counter() - t0
+        fb.options(use_graph=True)
+        fb.compute_device(dD.ptr, dG.ptr, stream.handle)
+        st = fb.stats()
+        stream.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fb.compute_device(dD.ptr, dG.ptr, stream.handle, synchronize=False)
+        stream.synchronize()
+        sb = (time.perf_counter() - t0) / steps
+        out[mode] = {"batch_builds_per_s": round(1.0 / sb, 3), "ms_per_batch": round(sb * 1e3, 4),
+                     "molecule_builds_per_s": round(len(sps) / sb, 1), "eris_per_s": round(st["eris"] / sb, 1),
+                     "eris_per_batch": st["eris"], "fp32_eri_share": round(st["eris_fp32"] / max(st["eris"], 1), 4),
+                     "autotune_s": round(t_tune, 2), "targets": fb.targets()}
+        if mode == "mixed":
+            out[mode]["tau"] = mixed_tau
+        fb.close()
+    # the same molecules one build per call (device-resident, FP64)
+    hs = [LI.Libint2Interface(sp, 1) for sp in sps]
+    bufs = [(H.DeviceArray.from_host(np.asfortranarray(D).ravel(order="F")), H.DeviceArray((sp.nbasis ** 2,)))
+            for sp, D in zip(sps, Ds)]
+    for h, (d, g) in zip(hs, bufs):
+        h.compute_2body_device(d.ptr, g.ptr, -0.5, stream.handle, synchronize=True)
+    t0 = time.perf_counter()
+    reps = max(1, steps // 4)
+    for _ in range(reps):
+        for h, (d, g) in zip(hs, bufs):
+            h.compute_2body_device(d.ptr, g.ptr, -0.5, stream.handle, synchronize=False)
+    stream.synchronize()
+    s1 = (time.perf_counter() - t0) / reps
+    out["one_per_call_fp64"] = {"ms_per_256": round(s1 * 1e3, 4), "molecule_builds_per_s": round(len(sps) / s1, 1)}
+    out["fp64_batch_speedup_vs_one_per_call"] = round(s1 / (out["fp64"]["ms_per_batch"] * 1e-3), 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,6 +169,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=4)
+    ap.add_argument("--no-mixed", action="store_true", help="skip the C20H42 mixed-precision SCF sweep")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 256-cluster batched build")
+    ap.add_argument("--taus", default="1e-4,1e-5,1e-6,1e-7")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 256 MiB device-to-device copy (profiler byte-counter calibration)")
     args = ap.parse_args()
@@ -177,6 +288,16 @@ def main():
         "serialized_ms_per_build": round(sum(v[0] for v in per_class.values()) / nprof, 4),
         "classes": classes,
     }
+    if not args.no_mixed:
+        from openlowdin_amd import wavefunction as WF
+        print("[bench] C20H42 FP64 vs mixed SCF sweep...", file=sys.stderr, flush=True)
+        out["mixed"] = mixed_sweep(S.alkane(20, "cc-pVDZ"), max(3, args.steps // 2),
+                                   [float(t) for t in args.taus.split(",")], H, LI, WF, FL, stream)
+    if not args.no_batch:
+        from openlowdin_amd import batch as B
+        print("[bench] 256-cluster batch...", file=sys.stderr, flush=True)
+        tau_b = (out.get("mixed") or {}).get("best_tau_within_1e-6") or 1e-6
+        out["batch"] = batch_bench(args.steps, H, B, S, tau_b, stream)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle sample)...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(sp, D, args.cpu_stride, st["eris"])

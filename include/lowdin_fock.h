@@ -72,11 +72,23 @@ void LibintInterface_compute_2body_direct(void* lint, double* dens, double* resu
  * the upper triangle of the column-major result (the caller symmetrises). */
 void LibintInterface_compute_coupling_direct(void* lint, void* olint, double* dens, double* result);
 
+/* Libint2Iface.cpp:3482-3503 (LibintInterface_compute_1body_ints).
+ * integral_kind 1: overlap, 2: kinetic (-1/2 nabla^2), 3: attraction to the
+ * particles added with add_particle (sum -z/|r - R|); normalised with norma,
+ * n x n.  Computed on the GPU. */
+void LibintInterface_compute_1body_ints(void* lint, int integral_kind, double* result);
+
 /* ------------------------------------------------ extensions (no reference
  * counterpart; used by this package's host code, tests and benchmarks) */
 
 /* Message of the last failure in this thread ("" if none). */
 const char* lowdin_fock_last_error(void);
+
+/* One-body matrices on the GPU for arbitrary point charges: kind 1 overlap,
+ * 2 kinetic (-1/2 nabla^2, unscaled by mass), 3 attraction sum_C -Z_C/|r - C|
+ * over the ncharge charges Z at xyz (3 per charge, bohr).  n x n, normalised
+ * with norma.  Returns 0 on success. */
+int lowdin_fock_1body(void* lint, int kind, int ncharge, const double* Z, const double* xyz, double* result);
 
 typedef struct {
   uint64_t quartets;        /* shell quartets evaluated by the last build */
@@ -84,6 +96,8 @@ typedef struct {
   uint64_t prim_quartets;   /* primitive quartets evaluated */
   double kernel_ms;         /* device time of the integral kernels (HIP events) */
   double total_ms;          /* wall time of the call */
+  uint64_t quartets_fp32;   /* of `quartets`, evaluated in FP32 (mixed precision) */
+  uint64_t eris_fp32;       /* of `eris`, evaluated in FP32 */
 } lowdin_fock_stats;
 
 /* Counters of the last compute_2body_direct / compute_coupling_direct call
@@ -115,6 +129,16 @@ int lowdin_fock_compute_2body_device(void* lint, const double* d_dens, double* d
 /* Record HIP events around every class launch of the intra build. */
 int lowdin_fock_time_launches(void* lint, int enable);
 
+/* Precision of the intra-species build.  mode 0: all quartets in FP64 (the
+ * default, reference parity).  mode 1: mixed -- a quartet whose bound
+ * Dnorm_abcd * Qn_ab * Qn_cd is below tau runs its integrals in FP32 with
+ * FP64 contraction and accumulation; the rest in FP64.  Dnorm_abcd is the
+ * largest |D| over the six density blocks the quartet touches, Qn the
+ * Schwarz bound of the normalised functions (max over the pair's
+ * components).  Classes above 300 Cartesian integrals per quartet (dd|dp),
+ * (dd|dd) always run in FP64.  Returns 0 on success. */
+int lowdin_fock_set_precision(void* lint, int mode, double tau);
+
 /* Run the class launches of the intra build one after another on one
  * stream instead of overlapping them on several (for per-class timing). */
 int lowdin_fock_serialize_launches(void* lint, int enable);
@@ -124,6 +148,48 @@ int lowdin_fock_serialize_launches(void* lint, int enable);
  * Returns the number of launches. */
 int lowdin_fock_launch_profile(void* lint, int cap, int* classes, float* ms, uint64_t* quartets,
                                uint64_t* prim_quartets);
+
+/* ------------------------------------------------------------ batches
+ * Many independent intra-species builds (the molecules of a batch, each a
+ * species handle with its shells added and init_2body_ints called) in one
+ * class launch per quartet class.  No reference counterpart: the reference
+ * builds one species per call (Libint2Iface.cpp:418-618); a batch gives
+ * every molecule exactly the result of that call.  The species handles must
+ * outlive the batch.  Densities and results are packed device arrays: the
+ * n_m x n_m block of molecule m starts at element offsets[m]. */
+void* lowdin_fock_batch_new(int nmol, void* const* lints);
+void lowdin_fock_batch_del(void* batch);
+
+/* Fills offsets[0..nmol] (may be NULL) and returns the packed length. */
+int64_t lowdin_fock_batch_layout(void* batch, int64_t* offsets);
+
+/* G_m = J[D_m] + factors[m] K[D_m] for every molecule (factors: host array
+ * of nmol).  Enqueued on `stream` (NULL: the library stream); with
+ * synchronize != 0 waits and updates the batch statistics.  No allocation
+ * once the first call has sized the batch; with the graph option the
+ * kernels of a build replay as one captured HIP graph. */
+int lowdin_fock_batch_compute_2body_device(void* batch, const double* d_dens, double* d_result,
+                                           const double* factors, void* stream, int synchronize);
+
+/* time_launches: HIP events around each class launch (not in graph mode);
+ * serialize: class launches on one stream; use_graph: capture once, replay. */
+int lowdin_fock_batch_options(void* batch, int time_launches, int serialize, int use_graph);
+
+int lowdin_fock_batch_get_stats(void* batch, lowdin_fock_stats* out);
+
+int lowdin_fock_batch_launch_profile(void* batch, int cap, int* classes, float* ms, uint64_t* quartets,
+                                     uint64_t* prim_quartets);
+
+/* Autotuner: for each class launch, the number of workgroups its tasks are
+ * cut into is chosen from {512, ..., 8192} by timing `reps` serialised
+ * builds per candidate on the given inputs.  Results are unchanged by the
+ * choice up to FP64 summation order. */
+int lowdin_fock_batch_autotune(void* batch, const double* d_dens, double* d_result, const double* factors,
+                               void* stream, int reps);
+
+/* The 21 per-class workgroup targets (class pair id cx*(cx+1)/2+cy over the
+ * pair classes ss, ps, pp, ds, dp, dd).  Returns 21. */
+int lowdin_fock_batch_targets(void* batch, int* targets);
 
 #ifdef __cplusplus
 }

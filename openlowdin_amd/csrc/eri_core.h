@@ -30,6 +30,38 @@ namespace lowdin {
 
 __host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
 
+// Global-address-space views of pointers read from the molecule descriptors
+// (a generic pointer loaded from memory would otherwise be accessed with
+// flat instructions).
+template <typename T>
+using gcptr = const __attribute__((address_space(1))) T*;
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+
+// Member-wise copy of a pair record read through a global view (a struct
+// copy from an address-space-qualified object has no implicit constructor).
+__device__ __forceinline__ PairInfo load_pair(gcptr<PairInfo> p) {
+  PairInfo r;
+  r.sa = p->sa;
+  r.sb = p->sb;
+  r.bfa = p->bfa;
+  r.bfb = p->bfb;
+  r.poff = p->poff;
+  r.nprim = p->nprim;
+  r.la = p->la;
+  r.lb = p->lb;
+  r.Q = p->Q;
+  r.ABx = p->ABx;
+  r.ABy = p->ABy;
+  r.ABz = p->ABz;
+  r.Qn = p->Qn;
+  return r;
+}
+
+__device__ __forceinline__ void global_add(gptr<double> p, double v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // 1/sqrt(x) for x > 0: hardware v_rsq_f64 refined by one Newton step.
 __device__ __forceinline__ double rsqrt_nr(double x) {
   double r = __builtin_amdgcn_rsq(x);
@@ -39,20 +71,24 @@ __device__ __forceinline__ double rsqrt_nr(double x) {
 // -------------------------------------------------------------- Boys F_m(T)
 // Taylor expansion of F_L around the nearest tabulated point, exp(-T) the
 // same way, then downward recursion F_m = (2T F_{m+1} + e^{-T}) / (2m+1).
-template <int L>
-__device__ __forceinline__ void boys_f64(double T, const double* __restrict__ tab, double* F) {
+// The table row of T_i holds e^{-T_i} at column 0 and F_L(T_i) .. F_{L+7}(T_i)
+// at columns FO .. FO+7; RS doubles per row.  The full table has RS =
+// kBoysCols, FO = 1 + L; a per-class copy in LDS keeps only the nine
+// columns a class reads (RS = 9, FO = 1).
+template <int L, int RS = kBoysCols, int FO = 1 + L, typename TP>
+__device__ __forceinline__ void boys_f64(double T, TP tab, double* F) {
   if (T < kBoysTmax) {
     const int i = (int)(T * kBoysInvH + 0.5);
     const double x = (double)i * kBoysH - T;
-    const double* r = tab + i * kBoysCols;
-    double s = r[1 + L + 7];
-    s = fma(s, x * (1.0 / 7.0), r[1 + L + 6]);
-    s = fma(s, x * (1.0 / 6.0), r[1 + L + 5]);
-    s = fma(s, x * (1.0 / 5.0), r[1 + L + 4]);
-    s = fma(s, x * (1.0 / 4.0), r[1 + L + 3]);
-    s = fma(s, x * (1.0 / 3.0), r[1 + L + 2]);
-    s = fma(s, x * (1.0 / 2.0), r[1 + L + 1]);
-    s = fma(s, x, r[1 + L + 0]);
+    const auto r = tab + i * RS;
+    double s = r[FO + 7];
+    s = fma(s, x * (1.0 / 7.0), r[FO + 6]);
+    s = fma(s, x * (1.0 / 6.0), r[FO + 5]);
+    s = fma(s, x * (1.0 / 5.0), r[FO + 4]);
+    s = fma(s, x * (1.0 / 4.0), r[FO + 3]);
+    s = fma(s, x * (1.0 / 3.0), r[FO + 2]);
+    s = fma(s, x * (1.0 / 2.0), r[FO + 1]);
+    s = fma(s, x, r[FO + 0]);
     F[L] = s;
     if (L > 0) {
       double e = 1.0;
@@ -107,10 +143,9 @@ struct EriClass;
 // primitive pairs (per lane, vector loads) form the outer loop and are read
 // once; the bra's primitive pairs (wave-uniform, scalar loads) the inner
 // one.  VRR accumulates [e0|f0] over all primitive quartets, then one HRR.
-template <int LA, int LB, int LC, int LD>
-__device__ __forceinline__ void contracted_quartet(const double* __restrict__ bprim, int nbp,
-                                                   const double* __restrict__ kprim, int nkp,
-                                                   const double* __restrict__ boys,
+template <int LA, int LB, int LC, int LD, int RS = kBoysCols, int FO = 1 + LA + LB + LC + LD, typename BP,
+          typename KP, typename TP>
+__device__ __forceinline__ void contracted_quartet_body(BP bprim, int nbp, KP kprim, int nkp, TP boys,
                                                    double ABx, double ABy, double ABz, double CDx,
                                                    double CDy, double CDz, double* out) {
   using C = EriClass<LA, LB, LC, LD>;
@@ -119,13 +154,13 @@ __device__ __forceinline__ void contracted_quartet(const double* __restrict__ bp
 #pragma unroll
   for (int k = 0; k < C::nacc; ++k) acc[k] = 0.0;
   for (int kp = 0; kp < nkp; ++kp) {
-    const double* kq = kprim + kp * kPrimPairStride;
+    const auto kq = kprim + kp * kPrimPairStride;
     const double q = kq[0], o2q = kq[1];
     const double Qx = kq[2], Qy = kq[3], Qz = kq[4];
     const double QCx = kq[5], QCy = kq[6], QCz = kq[7];
     const double Kcd = kq[8];
     for (int ip = 0; ip < nbp; ++ip) {
-      const double* bp = bprim + ip * kPrimPairStride;
+      const auto bp = bprim + ip * kPrimPairStride;
       const double p = bp[0], o2p = bp[1];
       const double PQx = bp[2] - Qx, PQy = bp[3] - Qy, PQz = bp[4] - Qz;
       const double rs = rsqrt_nr(p + q);
@@ -135,7 +170,7 @@ __device__ __forceinline__ void contracted_quartet(const double* __restrict__ bp
       const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
       const double pref = bp[8] * Kcd * rs;
       double F[L + 1];
-      boys_f64<L>(T, boys, F);
+      boys_f64<L, RS, FO>(T, boys, F);
 #pragma unroll
       for (int m = 0; m <= L; ++m) F[m] *= pref;
       C::template vrr<double>(F, bp[5], bp[6], bp[7], -rop * PQx, -rop * PQy, -rop * PQz, QCx, QCy, QCz,
@@ -145,10 +180,130 @@ __device__ __forceinline__ void contracted_quartet(const double* __restrict__ bp
   C::template hrr<double>(acc, ABx, ABy, ABz, CDx, CDy, CDz, out);
 }
 
+// FP32 Boys: fifth-order Taylor on the same grid (|dT| <= 0.05, error
+// ~3e-9 relative), exp(-T) the same way, downward recursion.
+template <int L>
+__device__ __forceinline__ void boys_f32(float T, const float* __restrict__ tab, float* F) {
+  if (T < (float)kBoysTmax) {
+    const int i = (int)(T * (float)kBoysInvH + 0.5f);
+    const float x = (float)i * (float)kBoysH - T;
+    const float* r = tab + i * kBoysCols;
+    float s = r[1 + L + 4];
+    s = fmaf(s, x * 0.25f, r[1 + L + 3]);
+    s = fmaf(s, x * (1.0f / 3.0f), r[1 + L + 2]);
+    s = fmaf(s, x * 0.5f, r[1 + L + 1]);
+    s = fmaf(s, x, r[1 + L + 0]);
+    F[L] = s;
+    if (L > 0) {
+      float e = 1.0f;
+      e = fmaf(e, x * 0.25f, 1.0f);
+      e = fmaf(e, x * (1.0f / 3.0f), 1.0f);
+      e = fmaf(e, x * 0.5f, 1.0f);
+      e = fmaf(e, x, 1.0f);
+      const float eT = r[0] * e;
+      const float t2 = 2.0f * T;
+#pragma unroll
+      for (int m = L - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + eT) * (1.0f / (2 * m + 1));
+    }
+  } else {
+    F[0] = 0.88622692545275801365f * __builtin_amdgcn_rsqf(T);
+    if (L > 0) {
+      const float o2t = 0.5f / T;
+      const float eT = __expf(-T);
+#pragma unroll
+      for (int m = 0; m < L; ++m) F[m + 1] = ((float)(2 * m + 1) * F[m] - eT) * o2t;
+    }
+  }
+}
+
+// FP32 contracted quartet (mixed precision): same loop structure as the
+// FP64 one over FP32 copies of the primitive-pair records; the integrals
+// come back as FP64 for the FP64 contraction and accumulation.
+template <int LA, int LB, int LC, int LD, typename BP, typename KP>
+__device__ __forceinline__ void contracted_quartet_f32_body(BP bprim, int nbp, KP kprim, int nkp,
+                                                       const float* __restrict__ boys, double ABx, double ABy,
+                                                       double ABz, double CDx, double CDy, double CDz,
+                                                       double* out) {
+  using C = EriClass<LA, LB, LC, LD>;
+  constexpr int L = LA + LB + LC + LD;
+  constexpr int NOUT = ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD);
+  float acc[C::nacc];
+#pragma unroll
+  for (int k = 0; k < C::nacc; ++k) acc[k] = 0.0f;
+  for (int kp = 0; kp < nkp; ++kp) {
+    const auto kq = kprim + kp * kPrimPairStride;
+    const float q = kq[0], o2q = kq[1];
+    const float Qx = kq[2], Qy = kq[3], Qz = kq[4];
+    const float QCx = kq[5], QCy = kq[6], QCz = kq[7];
+    const float Kcd = kq[8];
+    for (int ip = 0; ip < nbp; ++ip) {
+      const auto bp = bprim + ip * kPrimPairStride;
+      const float p = bp[0], o2p = bp[1];
+      const float PQx = bp[2] - Qx, PQy = bp[3] - Qy, PQz = bp[4] - Qz;
+      const float rs = __builtin_amdgcn_rsqf(p + q);
+      const float inv = rs * rs;
+      const float rop = q * inv, roq = p * inv;
+      const float rho = p * rop;
+      const float T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+      const float pref = bp[8] * Kcd * rs;
+      float F[L + 1];
+      boys_f32<L>(T, boys, F);
+#pragma unroll
+      for (int m = 0; m <= L; ++m) F[m] *= pref;
+      C::template vrr<float>(F, bp[5], bp[6], bp[7], -rop * PQx, -rop * PQy, -rop * PQz, QCx, QCy, QCz,
+                             roq * PQx, roq * PQy, roq * PQz, o2p, o2q, 0.5f * inv, rop, roq, acc);
+    }
+  }
+  float o[NOUT];
+  C::template hrr<float>(acc, (float)ABx, (float)ABy, (float)ABz, (float)CDx, (float)CDy, (float)CDz, o);
+#pragma unroll
+  for (int k = 0; k < NOUT; ++k) out[k] = (double)o[k];
+}
+
+// Quartet evaluation is inlined into every kernel of a class: an
+// out-of-line body for the largest classes doubled their scratch (the
+// integral array then lives in the caller's private segment as well) and
+// the runtime could not provision it at full occupancy.  The two largest
+// classes ((dd|dp), (dd|dd): single-primitive d shells in every workload
+// here) keep FP64 in mixed mode, which halves their code and compile time.
+constexpr int kMaxNaccFp32 = 300;
+
+template <int LA, int LB, int LC, int LD, int RS = kBoysCols, int FO = 1 + LA + LB + LC + LD, typename BP,
+          typename KP, typename TP>
+__device__ __forceinline__ void contracted_quartet(BP bprim, int nbp, KP kprim, int nkp, TP boys, double ABx,
+                                                   double ABy, double ABz, double CDx, double CDy, double CDz,
+                                                   double* out) {
+  contracted_quartet_body<LA, LB, LC, LD, RS, FO>(bprim, nbp, kprim, nkp, boys, ABx, ABy, ABz, CDx, CDy, CDz, out);
+}
+
+template <int LA, int LB, int LC, int LD>
+struct Fp32Allowed {
+  static constexpr bool value = EriClass<LA, LB, LC, LD>::nacc <= kMaxNaccFp32;
+};
+
+template <int LA, int LB, int LC, int LD, typename BP, typename KP>
+__device__ __forceinline__ void contracted_quartet_f32(BP bprim, int nbp, KP kprim, int nkp,
+                                                       const float* __restrict__ boys, double ABx, double ABy,
+                                                       double ABz, double CDx, double CDy, double CDz, double* out) {
+  contracted_quartet_f32_body<LA, LB, LC, LD>(bprim, nbp, kprim, nkp, boys, ABx, ABy, ABz, CDx, CDy, CDz, out);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
+}
+
+// Largest block infinity norm of D over the six shell blocks a quartet
+// touches (ab, cd, ac, bd, ad, bc).
+__device__ __forceinline__ double quartet_dnorm(gcptr<double> Dn, int ns, const PairInfo& P, const PairInfo& K) {
+  double dn = Dn[P.sa * ns + P.sb];
+  dn = fmax(dn, Dn[K.sa * ns + K.sb]);
+  dn = fmax(dn, Dn[P.sa * ns + K.sa]);
+  dn = fmax(dn, Dn[P.sb * ns + K.sb]);
+  dn = fmax(dn, Dn[P.sa * ns + K.sb]);
+  dn = fmax(dn, Dn[P.sb * ns + K.sa]);
+  return dn;
 }
 
 // ------------------------------------------------------------ J/K kernel
@@ -158,23 +313,24 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
   const int wid = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   if (wid >= a.ntasks) return;
   const Task t = a.tasks[wid];
+  const MolDesc& M = a.mols[t.flags];
+  const gcptr<PairInfo> pairs = (gcptr<PairInfo>)M.pairs;
+  const gcptr<double> prim = (gcptr<double>)M.prim;
+  const gcptr<float> primf = (gcptr<float>)M.primf;
+  const gcptr<double> D = (gcptr<double>)M.D;
+  const gcptr<double> nrm = (gcptr<double>)M.norma;
+  const gptr<double> G = (gptr<double>)M.Graw;
   const int lane = threadIdx.x & 63;
-  const PairInfo& P = a.bra[t.bra];
+  const PairInfo P = load_pair(pairs + M.off[a.cbra] + t.bra);
   const int j = t.j0 + lane;
   bool active = j < t.j1;
-  PairInfo K;
-  if (active) K = a.ket[j];
-  else K = P;
-  const int n = a.n, ns = a.nshell;
-  if (active && a.screen) {
-    const double* Dn = a.Dn;
-    double dn = Dn[P.sa * ns + P.sb];
-    dn = fmax(dn, Dn[K.sa * ns + K.sb]);
-    dn = fmax(dn, Dn[P.sa * ns + K.sa]);
-    dn = fmax(dn, Dn[P.sb * ns + K.sb]);
-    dn = fmax(dn, Dn[P.sa * ns + K.sb]);
-    dn = fmax(dn, Dn[P.sb * ns + K.sa]);
-    if (dn * P.Q * K.Q < a.eps) active = false;
+  const PairInfo K = active ? load_pair(pairs + M.off[a.cket] + j) : P;
+  const int n = M.n, ns = M.nshell;
+  bool lo = false;
+  if (active && (M.screen || M.mixed)) {
+    const double dn = quartet_dnorm((gcptr<double>)M.Dn, ns, P, K);
+    if (M.screen && dn * P.Q * K.Q < a.eps) active = false;
+    lo = Fp32Allowed<LA, LB, LC, LD>::value && M.mixed && dn * P.Qn * K.Qn < M.tau;
   }
   const uint64_t amask = __ballot(active);
   if (a.counts) {
@@ -183,9 +339,11 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
     unsigned int np = active ? (unsigned int)(P.nprim * K.nprim) : 0u;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) np += __shfl_xor(np, o, 64);
+    const uint64_t lmask = __ballot(active && lo);
     if (lane == 0) {
-      a.counts[2 * wid] = (unsigned int)__popcll(amask);
-      a.counts[2 * wid + 1] = np;
+      a.counts[3 * wid] = (unsigned int)__popcll(amask);
+      a.counts[3 * wid + 1] = np;
+      a.counts[3 * wid + 2] = (unsigned int)__popcll(lmask);
     }
   }
   double jab[NA * NB];
@@ -193,13 +351,18 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
   for (int k = 0; k < NA * NB; ++k) jab[k] = 0.0;
   if (active) {
     double out[NA * NB * NC * ND];
-    contracted_quartet<LA, LB, LC, LD>(a.prim + (size_t)P.poff * kPrimPairStride, P.nprim,
-                                       a.prim + (size_t)K.poff * kPrimPairStride, K.nprim, a.boys,
-                                       P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
+    if constexpr (Fp32Allowed<LA, LB, LC, LD>::value) {
+      if (lo)
+        contracted_quartet_f32<LA, LB, LC, LD>(primf + (size_t)P.poff * kPrimPairStride, P.nprim,
+                                               primf + (size_t)K.poff * kPrimPairStride, K.nprim, a.boysf,
+                                               P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
+    }
+    if (!lo)
+      contracted_quartet<LA, LB, LC, LD>(prim + (size_t)P.poff * kPrimPairStride, P.nprim,
+                                         prim + (size_t)K.poff * kPrimPairStride, K.nprim, a.boys,
+                                         P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
     const double deg = (P.sa != P.sb ? 2.0 : 1.0) * (K.sa != K.sb ? 2.0 : 1.0) *
                        ((a.same_class && j == t.bra) ? 1.0 : 2.0);
-    const double* D = a.D;
-    const double* nrm = a.norma;
     double na[NA], nb[NB], nc[NC], nd[ND];
 #pragma unroll
     for (int i = 0; i < NA; ++i) na[i] = nrm[P.bfa + i];
@@ -262,33 +425,32 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
             kad[ia * ND + id] += dbc[ib * NC + ic] * v;
             kbc[ib * NC + ic] += dad[ia * ND + id] * v;
           }
-    const double sj = deg, sk = deg * a.half_factor;
-    double* G = a.Graw;
+    const double sj = deg, sk = deg * M.half_factor;
 #pragma unroll
     for (int ic = 0; ic < NC; ++ic)
 #pragma unroll
       for (int id = 0; id < ND; ++id)
-        unsafeAtomicAdd(&G[(size_t)(K.bfa + ic) * n + K.bfb + id], jcd[ic * ND + id] * sj * nc[ic] * nd[id]);
+        global_add(G + (size_t)(K.bfa + ic) * n + K.bfb + id, jcd[ic * ND + id] * sj * nc[ic] * nd[id]);
 #pragma unroll
     for (int ia = 0; ia < NA; ++ia)
 #pragma unroll
       for (int ic = 0; ic < NC; ++ic)
-        unsafeAtomicAdd(&G[(size_t)(P.bfa + ia) * n + K.bfa + ic], kac[ia * NC + ic] * sk * na[ia] * nc[ic]);
+        global_add(G + (size_t)(P.bfa + ia) * n + K.bfa + ic, kac[ia * NC + ic] * sk * na[ia] * nc[ic]);
 #pragma unroll
     for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
       for (int id = 0; id < ND; ++id)
-        unsafeAtomicAdd(&G[(size_t)(P.bfb + ib) * n + K.bfb + id], kbd[ib * ND + id] * sk * nb[ib] * nd[id]);
+        global_add(G + (size_t)(P.bfb + ib) * n + K.bfb + id, kbd[ib * ND + id] * sk * nb[ib] * nd[id]);
 #pragma unroll
     for (int ia = 0; ia < NA; ++ia)
 #pragma unroll
       for (int id = 0; id < ND; ++id)
-        unsafeAtomicAdd(&G[(size_t)(P.bfa + ia) * n + K.bfb + id], kad[ia * ND + id] * sk * na[ia] * nd[id]);
+        global_add(G + (size_t)(P.bfa + ia) * n + K.bfb + id, kad[ia * ND + id] * sk * na[ia] * nd[id]);
 #pragma unroll
     for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
       for (int ic = 0; ic < NC; ++ic)
-        unsafeAtomicAdd(&G[(size_t)(P.bfb + ib) * n + K.bfa + ic], kbc[ib * NC + ic] * sk * nb[ib] * nc[ic]);
+        global_add(G + (size_t)(P.bfb + ib) * n + K.bfa + ic, kbc[ib * NC + ic] * sk * nb[ib] * nc[ic]);
 #pragma unroll
     for (int k = 0; k < NA * NB; ++k) jab[k] *= deg;
   }
@@ -299,8 +461,7 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
     for (int ib = 0; ib < NB; ++ib) {
       const double s = wave_sum(jab[ia * NB + ib]);
       if (lane == 0 && amask)
-        unsafeAtomicAdd(&a.Graw[(size_t)(P.bfa + ia) * n + P.bfb + ib],
-                        s * a.norma[P.bfa + ia] * a.norma[P.bfb + ib]);
+        global_add(G + (size_t)(P.bfa + ia) * n + P.bfb + ib, s * nrm[P.bfa + ia] * nrm[P.bfb + ib]);
     }
 }
 
@@ -311,26 +472,57 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
 // in row b), so those are accumulated with LDS atomics in two row strips
 // (NA + NB) x n held for the whole ket stream and flushed to HBM once, with
 // one global atomic per touched element.  J_cd, distinct for every ket, is
-// the only per-quartet global atomic.
+// the only per-quartet global atomic.  The bra's primitive-pair records
+// (at most kMaxStripPrims, guaranteed by the host) are staged in LDS once:
+// every lane reads the same record, an LDS broadcast.
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(256) void jk_strip_kernel(JKArgs a) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  constexpr bool kF32 = Fp32Allowed<LA, LB, LC, LD>::value;
   extern __shared__ double strip[];  // rows of a (NA x n), then rows of b (NB x n)
-  __shared__ unsigned int wg_q, wg_p;
+  __shared__ double bpd[kMaxStripPrims * kPrimPairStride];
+  __shared__ float bpf[kF32 ? kMaxStripPrims * kPrimPairStride : 1];
+#ifdef LOWDIN_LDS_BOYS
+  constexpr int LT = LA + LB + LC + LD;
+  __shared__ double btab[kBoysRows * 9];   // e^{-T_i}, F_LT .. F_LT+7 of this class
+#endif
+  __shared__ unsigned int wg_q, wg_p, wg_q32;
   const int tid = threadIdx.x;
   const Task t = a.tasks[blockIdx.x];
-  const PairInfo& P = a.bra[t.bra];
-  const int n = a.n, ns = a.nshell;
+  const MolDesc& M = a.mols[t.flags];
+  const gcptr<PairInfo> pairs = (gcptr<PairInfo>)M.pairs;
+  const gcptr<double> prim = (gcptr<double>)M.prim;
+  const gcptr<float> primf = (gcptr<float>)M.primf;
+  const gcptr<double> D = (gcptr<double>)M.D;
+  const gcptr<double> nrm = (gcptr<double>)M.norma;
+  const gptr<double> G = (gptr<double>)M.Graw;
+  const PairInfo P = load_pair(pairs + M.off[a.cbra] + t.bra);
+  const int n = M.n, ns = M.nshell;
   double* sa = strip;
   double* sb = strip + NA * n;
   for (int k = tid; k < (NA + NB) * n; k += blockDim.x) strip[k] = 0.0;
+  {
+    const gcptr<double> src = prim + (size_t)P.poff * kPrimPairStride;
+    for (int k = tid; k < P.nprim * kPrimPairStride; k += blockDim.x) bpd[k] = src[k];
+    if constexpr (kF32) {
+      if (M.mixed) {
+        const gcptr<float> srcf = primf + (size_t)P.poff * kPrimPairStride;
+        for (int k = tid; k < P.nprim * kPrimPairStride; k += blockDim.x) bpf[k] = srcf[k];
+      }
+    }
+  }
+#ifdef LOWDIN_LDS_BOYS
+  for (int k = tid; k < kBoysRows * 9; k += blockDim.x) {
+    const int r = k / 9, c = k - 9 * r;
+    btab[k] = a.boys[r * kBoysCols + (c == 0 ? 0 : LT + c)];
+  }
+#endif
   if (tid == 0) {
     wg_q = 0;
     wg_p = 0;
+    wg_q32 = 0;
   }
   __syncthreads();
-  const double* D = a.D;
-  const double* nrm = a.norma;
   double na[NA], nb[NB];
 #pragma unroll
   for (int i = 0; i < NA; ++i) na[i] = nrm[P.bfa + i];
@@ -344,26 +536,39 @@ __global__ __launch_bounds__(256) void jk_strip_kernel(JKArgs a) {
   double jab[NA * NB];
 #pragma unroll
   for (int k = 0; k < NA * NB; ++k) jab[k] = 0.0;
-  unsigned int myq = 0, myp = 0;
+  unsigned int myq = 0, myp = 0, myq32 = 0;
   const double sbra = (P.sa != P.sb ? 2.0 : 1.0);
+  const double hf = M.half_factor;
   for (int j = t.j0 + tid; j < t.j1; j += blockDim.x) {
-    const PairInfo K = a.ket[j];
-    if (a.screen) {
-      const double* Dn = a.Dn;
-      double dn = Dn[P.sa * ns + P.sb];
-      dn = fmax(dn, Dn[K.sa * ns + K.sb]);
-      dn = fmax(dn, Dn[P.sa * ns + K.sa]);
-      dn = fmax(dn, Dn[P.sb * ns + K.sb]);
-      dn = fmax(dn, Dn[P.sa * ns + K.sb]);
-      dn = fmax(dn, Dn[P.sb * ns + K.sa]);
-      if (dn * P.Q * K.Q < a.eps) continue;
+    const PairInfo K = load_pair(pairs + M.off[a.cket] + j);
+    bool lo = false;
+    if (M.screen || M.mixed) {
+      const double dn = quartet_dnorm((gcptr<double>)M.Dn, ns, P, K);
+      if (M.screen && dn * P.Q * K.Q < a.eps) continue;
+      lo = kF32 && M.mixed && dn * P.Qn * K.Qn < M.tau;
     }
     myq += 1;
     myp += (unsigned int)(P.nprim * K.nprim);
     double out[NA * NB * NC * ND];
-    contracted_quartet<LA, LB, LC, LD>(a.prim + (size_t)P.poff * kPrimPairStride, P.nprim,
-                                       a.prim + (size_t)K.poff * kPrimPairStride, K.nprim, a.boys,
-                                       P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
+    if constexpr (kF32) {
+      if (lo) {
+        myq32 += 1;
+        contracted_quartet_f32<LA, LB, LC, LD>((const float*)bpf, P.nprim,
+                                               primf + (size_t)K.poff * kPrimPairStride, K.nprim, a.boysf,
+                                               P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
+      }
+    }
+    if (!lo) {
+#ifdef LOWDIN_LDS_BOYS
+      contracted_quartet<LA, LB, LC, LD, 9, 1>((const double*)bpd, P.nprim,
+                                               prim + (size_t)K.poff * kPrimPairStride, K.nprim, (const double*)btab,
+                                               P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
+#else
+      contracted_quartet<LA, LB, LC, LD>((const double*)bpd, P.nprim,
+                                         prim + (size_t)K.poff * kPrimPairStride, K.nprim, a.boys,
+                                         P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
+#endif
+    }
     const double deg = sbra * (K.sa != K.sb ? 2.0 : 1.0) * ((a.same_class && j == t.bra) ? 1.0 : 2.0);
     double nc[NC], nd[ND];
 #pragma unroll
@@ -418,13 +623,11 @@ __global__ __launch_bounds__(256) void jk_strip_kernel(JKArgs a) {
             kad[ia * ND + id] += dbc[ib * NC + ic] * v;
             kbc[ib * NC + ic] += dad[ia * ND + id] * v;
           }
-    const double hf = a.half_factor;
-    double* G = a.Graw;
 #pragma unroll
     for (int ic = 0; ic < NC; ++ic)
 #pragma unroll
       for (int id = 0; id < ND; ++id)
-        unsafeAtomicAdd(&G[(size_t)(K.bfa + ic) * n + K.bfb + id], jcd[ic * ND + id] * nc[ic] * nd[id]);
+        global_add(G + (size_t)(K.bfa + ic) * n + K.bfb + id, jcd[ic * ND + id] * nc[ic] * nd[id]);
 #pragma unroll
     for (int ia = 0; ia < NA; ++ia) {
 #pragma unroll
@@ -449,23 +652,24 @@ __global__ __launch_bounds__(256) void jk_strip_kernel(JKArgs a) {
   if (a.counts) {
     atomicAdd(&wg_q, myq);
     atomicAdd(&wg_p, myp);
+    atomicAdd(&wg_q32, myq32);
   }
   __syncthreads();
   // flush: one global atomic per touched strip element (row norma applied here)
-  double* G = a.Graw;
   for (int r = 0; r < NA; ++r)
     for (int c = tid; c < n; c += blockDim.x) {
       const double v = sa[r * n + c];
-      if (v != 0.0) unsafeAtomicAdd(&G[(size_t)(P.bfa + r) * n + c], v * na[r]);
+      if (v != 0.0) global_add(G + (size_t)(P.bfa + r) * n + c, v * na[r]);
     }
   for (int r = 0; r < NB; ++r)
     for (int c = tid; c < n; c += blockDim.x) {
       const double v = sb[r * n + c];
-      if (v != 0.0) unsafeAtomicAdd(&G[(size_t)(P.bfb + r) * n + c], v * nb[r]);
+      if (v != 0.0) global_add(G + (size_t)(P.bfb + r) * n + c, v * nb[r]);
     }
   if (a.counts && tid == 0) {
-    a.counts[2 * blockIdx.x] = wg_q;
-    a.counts[2 * blockIdx.x + 1] = wg_p;
+    a.counts[3 * blockIdx.x] = wg_q;
+    a.counts[3 * blockIdx.x + 1] = wg_p;
+    a.counts[3 * blockIdx.x + 2] = wg_q32;
   }
 }
 
@@ -553,7 +757,7 @@ __global__ __launch_bounds__(256) void coupling_kernel(CouplingArgs a) {
 // (ab|ab) block, no primitive screening (Libint2Iface.cpp:3361-3404).
 template <int LA, int LB>
 __global__ __launch_bounds__(256) void schwarz_kernel(PairInfo* pairs, int npairs, const double* prim,
-                                                      const double* boys) {
+                                                      const double* boys, const double* norma) {
   constexpr int NA = ncart(LA), NB = ncart(LB), N2 = NA * NB;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npairs) return;
@@ -562,10 +766,19 @@ __global__ __launch_bounds__(256) void schwarz_kernel(PairInfo* pairs, int npair
   const double* pp = prim + (size_t)P.poff * kPrimPairStride;
   contracted_quartet<LA, LB, LA, LB>(pp, P.nprim, pp, P.nprim, boys, P.ABx, P.ABy, P.ABz, P.ABx,
                                      P.ABy, P.ABz, out);
-  double mx = 0.0;
+  double mx = 0.0, mxn = 0.0;
 #pragma unroll
-  for (int k = 0; k < N2; ++k) mx = fmax(mx, fabs(out[k * N2 + k]));
+  for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+    for (int ib = 0; ib < NB; ++ib) {
+      const int k = ia * NB + ib;
+      const double v = fabs(out[k * N2 + k]);
+      const double nn = norma[P.bfa + ia] * norma[P.bfb + ib];
+      mx = fmax(mx, v);
+      mxn = fmax(mxn, v * nn * nn);
+    }
   pairs[i].Q = sqrt(mx);
+  pairs[i].Qn = sqrt(mxn);
 }
 
 template <int LA, int LB, int LC, int LD>
@@ -578,7 +791,7 @@ static void launch_jk_t(const JKArgs& a, hipStream_t s) {
 template <int LA, int LB, int LC, int LD>
 static void launch_jk_strip_t(const JKArgs& a, hipStream_t s) {
   if (a.ntasks <= 0) return;
-  const size_t lds = (size_t)(ncart(LA) + ncart(LB)) * a.n * sizeof(double);
+  const size_t lds = (size_t)(ncart(LA) + ncart(LB)) * a.lds_n * sizeof(double);
   hipLaunchKernelGGL((jk_strip_kernel<LA, LB, LC, LD>), dim3(a.ntasks), dim3(256), lds, s, a);
 }
 
@@ -590,9 +803,11 @@ static void launch_cp_t(const CouplingArgs& a, hipStream_t s) {
 }
 
 template <int LA, int LB>
-static void launch_schwarz_t(PairInfo* pairs, int npairs, const double* prim, const double* boys, hipStream_t s) {
+static void launch_schwarz_t(PairInfo* pairs, int npairs, const double* prim, const double* boys, const double* norma,
+                             hipStream_t s) {
   if (npairs <= 0) return;
-  hipLaunchKernelGGL((schwarz_kernel<LA, LB>), dim3((npairs + 255) / 256), dim3(256), 0, s, pairs, npairs, prim, boys);
+  hipLaunchKernelGGL((schwarz_kernel<LA, LB>), dim3((npairs + 255) / 256), dim3(256), 0, s, pairs, npairs, prim, boys,
+                     norma);
 }
 
 }  // namespace lowdin

@@ -245,7 +245,8 @@ def main(outdir):
             fh.write("\n".join(inc))
         diag = (la, lb) == (lc, ld)
         schwfn = ("void launch_schwarz_%d%d(PairInfo* pairs, int npairs, const double* prim, const double* boys, "
-                  "hipStream_t s) { launch_schwarz_t<%d, %d>(pairs, npairs, prim, boys, s); }\n" % (la, lb, la, lb)
+                  "const double* norma, hipStream_t s) { launch_schwarz_t<%d, %d>(pairs, npairs, prim, boys, norma, s); }\n"
+                  % (la, lb, la, lb)
                   if diag else "")
         with open(os.path.join(gdir, "k_%s.hip" % k), "w") as fh:
             fh.write(KERNEL_TU % {"a": la, "b": lb, "c": lc, "d": ld, "k": k,
@@ -264,7 +265,8 @@ def main(outdir):
         lines.append("void launch_jks_%s(const JKArgs&, hipStream_t);" % k)
         lines.append("void launch_cp_%s(const CouplingArgs&, hipStream_t);" % k)
     for (la, lb) in pcs:
-        lines.append("void launch_schwarz_%d%d(PairInfo*, int, const double*, const double*, hipStream_t);" % (la, lb))
+        lines.append("void launch_schwarz_%d%d(PairInfo*, int, const double*, const double*, const double*, hipStream_t);"
+                     % (la, lb))
     lines.append("inline bool dispatch_jk(int la, int lb, int lc, int ld, const JKArgs& a, hipStream_t s) {")
     lines.append("  switch (la * 1000 + lb * 100 + lc * 10 + ld) {")
     for (la, lb, lc, ld) in classes:
@@ -280,10 +282,11 @@ def main(outdir):
     for (la, lb, lc, ld) in classes:
         lines.append("    case %d: launch_cp_%d%d%d%d(a, s); return true;" % (la * 1000 + lb * 100 + lc * 10 + ld, la, lb, lc, ld))
     lines.append("    default: return false;\n  }\n}")
-    lines.append("inline bool dispatch_schwarz(int la, int lb, PairInfo* p, int np, const double* prim, const double* boys, hipStream_t s) {")
+    lines.append("inline bool dispatch_schwarz(int la, int lb, PairInfo* p, int np, const double* prim, const double* boys, "
+                 "const double* norma, hipStream_t s) {")
     lines.append("  switch (la * 10 + lb) {")
     for (la, lb) in pcs:
-        lines.append("    case %d: launch_schwarz_%d%d(p, np, prim, boys, s); return true;" % (la * 10 + lb, la, lb))
+        lines.append("    case %d: launch_schwarz_%d%d(p, np, prim, boys, norma, s); return true;" % (la * 10 + lb, la, lb))
     lines.append("    default: return false;\n  }\n}")
     lines.append("}  // namespace lowdin")
     with open(os.path.join(gdir, "dispatch.h"), "w") as fh:

@@ -19,6 +19,7 @@ namespace lowdin {
 
 constexpr int kMaxL = 2;                 // s, p, d shells (Cartesian)
 constexpr int kPrimPairStride = 10;      // doubles per PrimPair record
+constexpr int kMaxStripPrims = 64;       // bra primitive pairs an LDS-strip workgroup stages
 constexpr double kPrimPairDrop = 69.0;   // drop primitive pairs with ab/p |AB|^2 > 69 (exp < 1e-30)
 
 // Boys table: F_m(T_i), m = 0..kBoysCols-2, at T_i = i * kBoysH, plus
@@ -47,8 +48,9 @@ struct PairInfo {   // one significant shell pair, canonical orientation
   int poff;         // first PrimPair record
   int nprim;        // surviving primitive pairs
   int la, lb;
-  double Q;         // raw Schwarz bound of the pair
+  double Q;         // raw Schwarz bound of the pair (the reference's screening convention)
   double ABx, ABy, ABz;
+  double Qn;        // normalised Schwarz bound: max over the pair's functions (mixed-precision test)
 };
 
 struct Task {       // one wavefront: bra pair x a run of <= 64 ket pairs

@@ -68,6 +68,12 @@ struct DevBuf {
 
 // ------------------------------------------------------------ process state
 constexpr int kSideStreams = 4;  // class launches overlap on these (default priority)
+// Kernels of the classes with total angular momentum >= 6 spill to a large
+// private (scratch) segment (up to ~15 KB per lane for (dd|dd)).  The
+// runtime provisions scratch per hardware queue for the device-wide wave
+// count, so such kernels run on one stream only (side stream 0): J/K class
+// launches, the Schwarz kernels and the coupling kernels alike.
+constexpr int kHeavyL = 6;
 
 struct Context {
   bool ready = false;
@@ -76,6 +82,7 @@ struct Context {
   hipEvent_t fork = nullptr, join[kSideStreams] = {};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   DevBuf<double> boys;
+  DevBuf<float> boysf;
   std::mutex mu;  // one process stream: calls are serialised
 };
 
@@ -123,6 +130,8 @@ static void ensure_context() {
   HIPCHK(hipEventCreate(&c.ev1));
   std::vector<double> tab = make_boys_table();
   c.boys.upload(tab.data(), tab.size(), c.stream);
+  std::vector<float> tabf(tab.begin(), tab.end());
+  c.boysf.upload(tabf.data(), tabf.size(), c.stream);
   HIPCHK(hipStreamSynchronize(c.stream));
   c.ready = true;
 }
@@ -144,6 +153,7 @@ struct PairSet {
   std::vector<double> prims;        // PrimPair records
   DevBuf<PairInfo> d_pairs;
   DevBuf<double> d_prims;
+  DevBuf<float> d_primsf;           // the same records in FP32
   bool q_ready = false;
 };
 
@@ -161,6 +171,8 @@ struct TaskSet {
   std::vector<size_t> order;   // launch order: estimated cost, largest first
   DevBuf<Task> d_tasks;
 };
+
+struct BuildPlan;
 
 struct Species {
   int stack_size = 0, species_id = 0;
@@ -181,7 +193,8 @@ struct Species {
 
   PairSet sig;        // significant pairs (intra builds), with Q
   PairSet all;        // every s1 <= s2 pair (coupling builds)
-  TaskSet intra;
+  uint64_t sig_gen = 0;   // bumps whenever sig is rebuilt
+  BuildPlan* own = nullptr;  // the one-molecule build plan of this species
   uint64_t uid = 0;   // process-unique identity (keys the coupling task cache)
   struct CouplingKey {
     uint64_t gen_self, gen_other;
@@ -189,22 +202,62 @@ struct Species {
   };
   std::map<uint64_t, CouplingKey> coupling_tasks;  // keyed by the other species' uid
 
-  DevBuf<double> d_D, d_Dn, d_Graw, d_G, d_Dother, d_R;
+  DevBuf<double> d_D, d_G, d_Dother, d_R;
+  lowdin_fock_stats stats{};
+  int mixed = 0;            // mixed precision: FP32 quartets below tau
+  double tau = 0.0;
+
+  ~Species();
+};
+
+// The class launches of an intra-species build over one or many species
+// (the molecules of a batch): one launch per quartet class, whose tasks carry
+// the index of their molecule (Task.flags) into a device array of MolDesc.
+// Densities and results are packed: molecule m's n_m x n_m block starts at
+// off[m] elements.
+constexpr int kNumClassPairs = 21;
+struct BuildPlan {
+  std::vector<Species*> mols;
+  std::vector<uint64_t> sig_gen;        // each molecule's sig_gen when the tasks were built
+  bool valid = false;
+  TaskSet ts;
+  std::vector<size_t> off, shoff;       // packed n^2 / nshell^2 offsets; last = total
+  int nmax = 0, nshmax = 0;
+  bool any_dn = false;                  // some molecule screens or runs mixed
+  std::vector<MolDesc> desc, desc_dev;  // wanted / on the device
+  std::vector<SymDesc> sym, sym_dev;
+  DevBuf<MolDesc> d_desc;
+  DevBuf<SymDesc> d_sym;
+  DevBuf<double> d_Graw, d_Dn;
   DevBuf<unsigned long long> d_cnt;
   DevBuf<unsigned int> d_task_counts;
+  uint64_t layout = 0;                  // bumps whenever tasks or buffers move
+  int target_wg[kNumClassPairs];        // workgroups a strip launch aims at (autotuned)
   lowdin_fock_stats stats{};
-  // per class launch of the last intra build: device ms, quartets, primitive quartets
-  std::vector<float> launch_ms;
+  std::vector<float> launch_ms;         // per class launch of the last build
   std::vector<unsigned long long> launch_cnt;
   std::vector<hipEvent_t> launch_ev;
   bool time_launches = false;
-  bool serialize = false;   // all class launches on one side stream (clean per-class timing)
+  bool serialize = false;               // all class launches on one side stream (clean per-class timing)
+  bool use_graph = false;
+  hipGraphExec_t graph = nullptr;
+  uint64_t graph_layout = ~0ull;
+  hipStream_t graph_stream = nullptr;
+  bool graph_any_dn = false;
 
-  ~Species() {
-    for (auto& kv : coupling_tasks) delete kv.second.tasks;
+  BuildPlan() {
+    for (int k = 0; k < kNumClassPairs; ++k) target_wg[k] = 2048;
+  }
+  ~BuildPlan() {
     for (hipEvent_t e : launch_ev) (void)hipEventDestroy(e);
+    if (graph) (void)hipGraphExecDestroy(graph);
   }
 };
+
+Species::~Species() {
+  for (auto& kv : coupling_tasks) delete kv.second.tasks;
+  delete own;
+}
 
 // N(alpha, l) = sqrt(2^l (2 alpha)^(l+3/2) / (pi^(3/2) (2l-1)!!)), the
 // normalisation of x^l exp(-alpha r^2) that libint2 embeds in raw
@@ -252,6 +305,7 @@ static void build_pairset(const Species& s, const std::vector<std::pair<int, int
     P.la = A.l;
     P.lb = B.l;
     P.Q = 0.0;
+    P.Qn = 0.0;
     P.ABx = A.x - B.x;
     P.ABy = A.y - B.y;
     P.ABz = A.z - B.z;
@@ -289,23 +343,65 @@ static void build_pairset(const Species& s, const std::vector<std::pair<int, int
   for (int c = 0; c < kNumPairClasses; ++c) ps.off[c + 1] += ps.off[c];
   ps.d_pairs.upload(ps.pairs.data(), ps.pairs.size(), st);
   ps.d_prims.upload(ps.prims.data(), ps.prims.size(), st);
+  std::vector<float> pf(ps.prims.begin(), ps.prims.end());
+  ps.d_primsf.upload(pf.data(), pf.size(), st);
+  HIPCHK(hipStreamSynchronize(st));
   ps.valid = true;
   ps.q_ready = false;
 }
 
-static void compute_schwarz(PairSet& ps, hipStream_t st) {
+// After Q is known: within each pair class order pairs by primitive count
+// (descending) then Q (descending), so that lanes of a wavefront run equal
+// primitive loops and the FP32 (small-bound) quartets of mixed precision
+// form contiguous runs.  Rewrites the primitive-pair records in that order.
+static void resort_by_q(PairSet& ps, hipStream_t st) {
+  std::vector<size_t> idx(ps.pairs.size());
+  for (size_t k = 0; k < idx.size(); ++k) idx[k] = k;
+  std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) {
+    const PairInfo& a = ps.pairs[x];
+    const PairInfo& b = ps.pairs[y];
+    const int ca = pair_class(a.la, a.lb), cb = pair_class(b.la, b.lb);
+    if (ca != cb) return ca < cb;
+    if (a.nprim != b.nprim) return a.nprim > b.nprim;
+    return a.Q > b.Q;
+  });
+  std::vector<PairInfo> pairs;
+  std::vector<double> prims;
+  pairs.reserve(ps.pairs.size());
+  prims.reserve(ps.prims.size());
+  for (size_t k : idx) {
+    PairInfo P = ps.pairs[k];
+    const size_t o = (size_t)P.poff * kPrimPairStride;
+    P.poff = (int)(prims.size() / kPrimPairStride);
+    prims.insert(prims.end(), ps.prims.begin() + o, ps.prims.begin() + o + (size_t)P.nprim * kPrimPairStride);
+    pairs.push_back(P);
+  }
+  ps.pairs.swap(pairs);
+  ps.prims.swap(prims);
+  ps.d_pairs.upload(ps.pairs.data(), ps.pairs.size(), st);
+  ps.d_prims.upload(ps.prims.data(), ps.prims.size(), st);
+  std::vector<float> pf(ps.prims.begin(), ps.prims.end());
+  ps.d_primsf.upload(pf.data(), pf.size(), st);
+  HIPCHK(hipStreamSynchronize(st));
+}
+
+// Runs on the heavy-kernel stream (the (dd|dd) diagonal spills); the pair
+// records were uploaded and their stream drained by build_pairset.
+static void compute_schwarz(PairSet& ps, const double* d_norma, hipStream_t st) {
   Context& c = ctx();
+  HIPCHK(hipStreamSynchronize(st));
+  hipStream_t hs = c.side[0];
   for (int pc = 0; pc < kNumPairClasses; ++pc) {
     const int n = ps.off[pc + 1] - ps.off[pc];
     if (n == 0) continue;
     if (!dispatch_schwarz(kPairClassL[pc][0], kPairClassL[pc][1], ps.d_pairs.p + ps.off[pc], n, ps.d_prims.p,
-                          c.boys.p, st))
+                          c.boys.p, d_norma, hs))
       throw std::runtime_error("no Schwarz kernel for pair class");
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(ps.pairs.data(), ps.d_pairs.p, ps.pairs.size() * sizeof(PairInfo), hipMemcpyDeviceToHost,
-                        st));
-  HIPCHK(hipStreamSynchronize(st));
+                        hs));
+  HIPCHK(hipStreamSynchronize(hs));
   ps.q_ready = true;
 }
 
@@ -334,72 +430,133 @@ static void sort_by_cost(std::vector<Task>& t, size_t from, const PairInfo* bras
 // Largest LDS row-strip footprint a J/K workgroup may use.
 constexpr size_t kStripLdsMax = 65536;
 
-// Intra-species tasks.  Where the bra's two row strips fit in LDS, a task is
-// one workgroup: a bra pair and a run of up to `chunk` kets (chunk sized so
-// that a class launch has at least ~2048 workgroups); otherwise a task is
-// one wavefront of up to 64 kets with direct global accumulation.
-static void build_intra_tasks(Species& s, hipStream_t st) {
-  TaskSet& ts = s.intra;
+static inline int class_pair_id(int cx, int cy) { return cx * (cx + 1) / 2 + cy; }
+
+static void ensure_sig(Species& s, hipStream_t st) {
+  ensure_device_basis(s, st);
+  if (!s.sig.valid) {
+    build_pairset(s, s.pairlist, s.sig, st);
+    compute_schwarz(s.sig, s.d_norma.p, st);
+    resort_by_q(s.sig, st);
+    s.sig_gen++;
+  }
+}
+
+// Tasks of a plan, one class launch per (bra class, ket class <= bra class)
+// present in any molecule.  Where the bra's two row strips fit in LDS (at
+// the largest n of the plan) a task is one workgroup: a bra pair and a run of
+// kets, the run length chosen so that the launch has about target_wg
+// workgroups; otherwise a task is one wavefront of up to 64 kets with direct
+// global accumulation.  Within a launch tasks run in order of estimated cost.
+static void build_plan_tasks(BuildPlan& p, hipStream_t st) {
+  TaskSet& ts = p.ts;
   ts.tasks.clear();
   ts.launches.clear();
-  const PairSet& ps = s.sig;
+  const int nm = (int)p.mols.size();
+  p.off.assign(nm + 1, 0);
+  p.shoff.assign(nm + 1, 0);
+  p.nmax = p.nshmax = 0;
+  p.any_dn = false;
+  for (int m = 0; m < nm; ++m) {
+    const Species& s = *p.mols[m];
+    const int ns = (int)s.shells.size();
+    p.off[m + 1] = p.off[m] + (size_t)s.nbasis * s.nbasis;
+    p.shoff[m + 1] = p.shoff[m] + (size_t)ns * ns;
+    p.nmax = std::max(p.nmax, s.nbasis);
+    p.nshmax = std::max(p.nshmax, ns);
+  }
+  std::vector<double> cost;
+  auto task_cost = [&](const Task& t, int cx, int cy) {
+    const PairSet& ps = p.mols[t.flags]->sig;
+    return (double)ps.pairs[ps.off[cx] + t.bra].nprim * ps.pairs[ps.off[cy] + t.j0].nprim * (t.j1 - t.j0);
+  };
   for (int cx = 0; cx < kNumPairClasses; ++cx) {
     for (int cy = 0; cy <= cx; ++cy) {
-      const int nx = ps.off[cx + 1] - ps.off[cx], ny = ps.off[cy + 1] - ps.off[cy];
-      if (nx == 0 || ny == 0) continue;
-      const PairInfo* bras = ps.pairs.data() + ps.off[cx];
-      const PairInfo* kets = ps.pairs.data() + ps.off[cy];
-      const size_t from = ts.tasks.size();
       const int la = kPairClassL[cx][0], lb = kPairClassL[cx][1];
-      const bool strip = (size_t)(ncart_h(la) + ncart_h(lb)) * s.nbasis * sizeof(double) <= kStripLdsMax;
-      if (strip) {
-        long total = 0;
-        for (int i = 0; i < nx; ++i) total += (cx == cy) ? i + 1 : ny;
-        long chunk = (total + 2047) / 2048;
-        chunk = std::max(256L, std::min(8192L, ((chunk + 255) / 256) * 256));
+      const int lc = kPairClassL[cy][0], ld = kPairClassL[cy][1];
+      long total = 0;
+      int maxbra = 0;   // largest bra primitive-pair count (staged in LDS by the strip kernel)
+      for (int m = 0; m < nm; ++m) {
+        const PairSet& ps = p.mols[m]->sig;
+        const int nx = ps.off[cx + 1] - ps.off[cx], ny = ps.off[cy + 1] - ps.off[cy];
+        if (nx && ny) {
+          total += (cx == cy) ? (long)nx * (nx + 1) / 2 : (long)nx * ny;
+          for (int i = ps.off[cx]; i < ps.off[cx + 1]; ++i) maxbra = std::max(maxbra, ps.pairs[i].nprim);
+        }
+      }
+      if (total == 0) continue;
+      const bool strip = (size_t)(ncart_h(la) + ncart_h(lb)) * p.nmax * sizeof(double) <= kStripLdsMax &&
+                         maxbra <= kMaxStripPrims;
+      const size_t from = ts.tasks.size();
+      // run length: at least enough kets to amortise zeroing and flushing the
+      // strips (which grow with n), at most 8192
+      const long cmin = std::max(64L, std::min(256L, (long)((p.nmax / 2 + 63) / 64) * 64));
+      long chunk = (total + p.target_wg[class_pair_id(cx, cy)] - 1) / p.target_wg[class_pair_id(cx, cy)];
+      chunk = std::max(cmin, std::min(8192L, ((chunk + 63) / 64) * 64));
+      for (int m = 0; m < nm; ++m) {
+        const PairSet& ps = p.mols[m]->sig;
+        const int nx = ps.off[cx + 1] - ps.off[cx], ny = ps.off[cy + 1] - ps.off[cy];
+        if (nx == 0 || ny == 0) continue;
+        const PairInfo* kets = ps.pairs.data() + ps.off[cy];
         for (int i = 0; i < nx; ++i) {
           const int nk = (cx == cy) ? i + 1 : ny;
-          const int pieces = (int)((nk + chunk - 1) / chunk);
-          for (int p = 0; p < pieces; ++p) {
-            const int j0 = (int)((long)nk * p / pieces), j1 = (int)((long)nk * (p + 1) / pieces);
-            if (j1 > j0) ts.tasks.push_back(Task{i, j0, j1, 1});
+          if (strip) {
+            const int pieces = (int)((nk + chunk - 1) / chunk);
+            for (int q = 0; q < pieces; ++q) {
+              const int j0 = (int)((long)nk * q / pieces), j1 = (int)((long)nk * (q + 1) / pieces);
+              if (j1 > j0) ts.tasks.push_back(Task{i, j0, j1, m});
+            }
+          } else {
+            emit_tasks(ts.tasks, i, 0, nk, kets, m);
           }
         }
-      } else {
-        for (int i = 0; i < nx; ++i) emit_tasks(ts.tasks, i, 0, cx == cy ? i + 1 : ny, kets, 0);
       }
-      sort_by_cost(ts.tasks, from, bras, kets);
-      ts.launches.push_back({la, lb, kPairClassL[cy][0], kPairClassL[cy][1], cx, cy, 1, from,
-                             ts.tasks.size() - from, strip ? 1 : 0});
-    }
-  }
-  // estimated cost of each class launch: primitive quartets x class weight
-  std::vector<double> cost(ts.launches.size(), 0.0);
-  for (size_t k = 0; k < ts.launches.size(); ++k) {
-    const TaskSet::Launch& L = ts.launches[k];
-    const PairInfo* bras = ps.pairs.data() + ps.off[L.cbra];
-    const PairInfo* kets = ps.pairs.data() + ps.off[L.cket];
-    const double w = 1.0 + (L.la + L.lb + L.lc + L.ld) * (L.la + L.lb + L.lc + L.ld);
-    for (size_t t = L.off; t < L.off + L.count; ++t) {
-      const Task& tk = ts.tasks[t];
-      cost[k] += w * (double)bras[tk.bra].nprim * kets[tk.j0].nprim * (tk.j1 - tk.j0);
+      std::stable_sort(ts.tasks.begin() + from, ts.tasks.end(),
+                       [&](const Task& x, const Task& y) { return task_cost(x, cx, cy) > task_cost(y, cx, cy); });
+      ts.launches.push_back({la, lb, lc, ld, cx, cy, 1, from, ts.tasks.size() - from, strip ? 1 : 0});
+      // estimated cost of the launch: primitive quartets x class weight
+      const double w = 1.0 + (la + lb + lc + ld) * (la + lb + lc + ld);
+      double c = 0.0;
+      for (size_t t = from; t < ts.tasks.size(); ++t) c += w * task_cost(ts.tasks[t], cx, cy);
+      cost.push_back(c);
     }
   }
   ts.order.resize(ts.launches.size());
   for (size_t k = 0; k < ts.order.size(); ++k) ts.order[k] = k;
   std::stable_sort(ts.order.begin(), ts.order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
   ts.d_tasks.upload(ts.tasks.data(), ts.tasks.size(), st);
+  const size_t nl = ts.launches.size();
+  p.d_cnt.ensure(3 * std::max<size_t>(nl, 1));
+  p.d_task_counts.ensure(3 * std::max<size_t>(ts.tasks.size(), 1));
+  p.d_Graw.ensure(std::max<size_t>(p.off[nm], 1));
+  p.d_Dn.ensure(std::max<size_t>(p.shoff[nm], 1));
+  p.d_desc.ensure(std::max(nm, 1));
+  p.d_sym.ensure(std::max(nm, 1));
+  p.desc_dev.clear();
+  p.sym_dev.clear();
+  p.sig_gen.resize(nm);
+  for (int m = 0; m < nm; ++m) p.sig_gen[m] = p.mols[m]->sig_gen;
+  HIPCHK(hipStreamSynchronize(st));
+  p.layout++;
   ts.valid = true;
+  p.valid = true;
 }
 
-static void ensure_sig(Species& s, hipStream_t st) {
-  ensure_device_basis(s, st);
-  if (!s.sig.valid) {
-    build_pairset(s, s.pairlist, s.sig, st);
-    compute_schwarz(s.sig, st);
-    s.intra.valid = false;
+static void ensure_plan(BuildPlan& p, hipStream_t st) {
+  bool stale = !p.valid || p.sig_gen.size() != p.mols.size();
+  for (size_t m = 0; m < p.mols.size(); ++m) {
+    ensure_sig(*p.mols[m], st);
+    if (!stale && p.sig_gen[m] != p.mols[m]->sig_gen) stale = true;
   }
-  if (!s.intra.valid) build_intra_tasks(s, st);
+  if (stale) build_plan_tasks(p, st);
+}
+
+static BuildPlan& own_plan(Species& s) {
+  if (!s.own) {
+    s.own = new BuildPlan();
+    s.own->mols.push_back(&s);
+  }
+  return *s.own;
 }
 
 static void ensure_all(Species& s, hipStream_t st) {
@@ -462,98 +619,164 @@ static double now_ms() {
 }
 
 // ------------------------------------------------------------ operations
-// Enqueue one intra-species build on stream st: D (device, n x n) -> G
-// (device, n x n).  Nothing is allocated once the species' buffers exist.
-static void enqueue_2body(Species& s, const double* dD, double* dG, double factor, hipStream_t st) {
+static void ensure_launch_events(BuildPlan& p) {
+  const size_t nl = p.ts.launches.size();
+  while (p.launch_ev.size() < 2 * nl) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    p.launch_ev.push_back(e);
+  }
+}
+
+// The kernels of one build, in stream order on st: Dnorm, zero G_raw, the
+// class launches fanned out over the side streams (largest first, so that
+// small latency-bound classes overlap the large ones), per-launch counter
+// reduction, symmetrisation.  Capturable: no memset, no allocation, no
+// host synchronisation.
+static void enqueue_plan_kernels(BuildPlan& p, hipStream_t st, bool timed) {
   Context& c = ctx();
-  const int n = s.nbasis, ns = (int)s.shells.size();
-  const size_t nn = (size_t)n * n;
-  ensure_sig(s, st);
-  const bool screen = s.is_electron;
-  if (screen) {
-    s.d_Dn.ensure((size_t)ns * ns);
-    launch_dnorm(dD, n, s.d_shells.p, ns, s.d_Dn.p, st);
-  }
-  s.d_Graw.ensure(nn);
-  HIPCHK(hipMemsetAsync(s.d_Graw.p, 0, nn * sizeof(double), st));
-  const size_t nl = s.intra.launches.size();
-  s.d_cnt.ensure(2 * std::max<size_t>(nl, 1));
-  s.d_task_counts.ensure(2 * std::max<size_t>(s.intra.tasks.size(), 1));
-  if (s.time_launches && s.launch_ev.size() < 2 * nl) {
-    while (s.launch_ev.size() < 2 * nl) {
-      hipEvent_t e;
-      HIPCHK(hipEventCreate(&e));
-      s.launch_ev.push_back(e);
-    }
-  }
-  // Class launches fan out over the side streams, largest first, so that
-  // small latency-bound classes overlap the large ones.
-  HIPCHK(hipEventRecord(c.ev0, st));
+  const int nm = (int)p.mols.size();
+  if (p.any_dn) launch_dnorm_batch(p.d_desc.p, nm, p.nshmax, st);
+  launch_fill_zero(p.d_Graw.p, p.off[nm], st);
+  const size_t nl = p.ts.launches.size();
   HIPCHK(hipEventRecord(c.fork, st));
   for (int q = 0; q < kSideStreams; ++q) HIPCHK(hipStreamWaitEvent(c.side[q], c.fork, 0));
   for (size_t r = 0; r < nl; ++r) {
-    const size_t k = s.intra.order[r];
-    const TaskSet::Launch& L = s.intra.launches[k];
-    hipStream_t ls = c.side[s.serialize ? 0 : r % kSideStreams];
+    const size_t k = p.ts.order[r];
+    const TaskSet::Launch& L = p.ts.launches[k];
+    const bool heavy = L.la + L.lb + L.lc + L.ld >= kHeavyL;
+    hipStream_t ls = c.side[(p.serialize || heavy) ? 0 : r % kSideStreams];
     JKArgs a{};
-    a.tasks = s.intra.d_tasks.p + L.off;
+    a.tasks = p.ts.d_tasks.p + L.off;
     a.ntasks = (int)L.count;
-    a.bra = s.sig.d_pairs.p + s.sig.off[L.cbra];
-    a.ket = s.sig.d_pairs.p + s.sig.off[L.cket];
-    a.prim = s.sig.d_prims.p;
-    a.D = dD;
-    a.Dn = s.d_Dn.p;
-    a.norma = s.d_norma.p;
+    a.mols = p.d_desc.p;
     a.boys = c.boys.p;
-    a.Graw = s.d_Graw.p;
-    a.counts = s.d_task_counts.p + 2 * L.off;
-    a.n = n;
-    a.nshell = ns;
-    a.screen = screen ? 1 : 0;
+    a.boysf = c.boysf.p;
+    a.counts = p.d_task_counts.p + 3 * L.off;
+    a.cbra = L.cbra;
+    a.cket = L.cket;
     a.same_class = L.cbra == L.cket ? 1 : 0;
+    a.lds_n = p.nmax;
     a.eps = 2.220446049250313080847e-16;  // DBL_EPSILON, Libint2Iface.h:269-271
-    a.half_factor = 0.5 * factor;
-    if (s.time_launches) HIPCHK(hipEventRecord(s.launch_ev[2 * k], ls));
+    if (timed) HIPCHK(hipEventRecord(p.launch_ev[2 * k], ls));
     const bool ok = L.strip ? dispatch_jk_strip(L.la, L.lb, L.lc, L.ld, a, ls) : dispatch_jk(L.la, L.lb, L.lc, L.ld, a, ls);
     if (!ok) throw std::runtime_error("no J/K kernel for class");
-    if (s.time_launches) HIPCHK(hipEventRecord(s.launch_ev[2 * k + 1], ls));
+    if (timed) HIPCHK(hipEventRecord(p.launch_ev[2 * k + 1], ls));
   }
   for (int q = 0; q < kSideStreams; ++q) {
     HIPCHK(hipEventRecord(c.join[q], c.side[q]));
     HIPCHK(hipStreamWaitEvent(st, c.join[q], 0));
   }
   for (size_t k = 0; k < nl; ++k) {
-    const TaskSet::Launch& L = s.intra.launches[k];
-    launch_reduce_counts(s.d_task_counts.p + 2 * L.off, (int)L.count, s.d_cnt.p + 2 * k, st);
+    const TaskSet::Launch& L = p.ts.launches[k];
+    launch_reduce_counts(p.d_task_counts.p + 3 * L.off, (int)L.count, p.d_cnt.p + 3 * k, st);
   }
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(c.ev1, st));
-  launch_symmetrize(s.d_Graw.p, n, 1.0, dG, st);
+  launch_symmetrize_batch(p.d_sym.p, nm, p.nmax, st);
   HIPCHK(hipGetLastError());
 }
 
-// After the stream has drained: fold the counters and event times into stats.
-static void collect_2body_stats(Species& s, hipStream_t st, double t0) {
+// Enqueue one build of plan p on stream st: packed densities dD -> packed
+// results dG, factors[m] the exchange factor of molecule m.  Nothing is
+// allocated once the plan's buffers exist; the molecule descriptors are
+// re-uploaded only when a pointer, factor or precision setting changed.
+static void enqueue_plan(BuildPlan& p, const double* dD, double* dG, const double* factors, hipStream_t st) {
   Context& c = ctx();
-  const size_t nl = s.intra.launches.size();
-  std::vector<unsigned long long> cnt(2 * std::max<size_t>(nl, 1));
-  HIPCHK(hipMemcpyAsync(cnt.data(), s.d_cnt.p, cnt.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+  ensure_plan(p, st);
+  const int nm = (int)p.mols.size();
+  p.desc.resize(nm);
+  p.sym.resize(nm);
+  p.any_dn = false;
+  for (int m = 0; m < nm; ++m) {
+    Species& s = *p.mols[m];
+    MolDesc& M = p.desc[m];
+    std::memset(&M, 0, sizeof(M));
+    M.pairs = s.sig.d_pairs.p;
+    M.prim = s.sig.d_prims.p;
+    M.primf = s.sig.d_primsf.p;
+    M.D = dD + p.off[m];
+    M.Dn = p.d_Dn.p + p.shoff[m];
+    M.shells = s.d_shells.p;
+    M.norma = s.d_norma.p;
+    M.Graw = p.d_Graw.p + p.off[m];
+    for (int k = 0; k <= kNumPairClasses; ++k) M.off[k] = s.sig.off[k];
+    M.n = s.nbasis;
+    M.nshell = (int)s.shells.size();
+    M.screen = s.is_electron ? 1 : 0;
+    M.mixed = s.mixed;
+    M.half_factor = 0.5 * factors[m];
+    M.tau = s.tau;
+    p.any_dn = p.any_dn || M.screen || M.mixed;
+    SymDesc& Y = p.sym[m];
+    std::memset(&Y, 0, sizeof(Y));
+    Y.Graw = M.Graw;
+    Y.G = dG + p.off[m];
+    Y.n = s.nbasis;
+  }
+  if (p.desc_dev.size() != p.desc.size() || std::memcmp(p.desc_dev.data(), p.desc.data(), nm * sizeof(MolDesc))) {
+    HIPCHK(hipMemcpyAsync(p.d_desc.p, p.desc.data(), nm * sizeof(MolDesc), hipMemcpyHostToDevice, st));
+    p.desc_dev = p.desc;
+  }
+  if (p.sym_dev.size() != p.sym.size() || std::memcmp(p.sym_dev.data(), p.sym.data(), nm * sizeof(SymDesc))) {
+    HIPCHK(hipMemcpyAsync(p.d_sym.p, p.sym.data(), nm * sizeof(SymDesc), hipMemcpyHostToDevice, st));
+    p.sym_dev = p.sym;
+  }
+  const bool timed = p.time_launches && !p.use_graph;
+  if (timed) ensure_launch_events(p);
+  HIPCHK(hipEventRecord(c.ev0, st));
+  if (p.use_graph) {
+    if (!p.graph || p.graph_layout != p.layout || p.graph_stream != st || p.graph_any_dn != p.any_dn) {
+      if (p.graph) HIPCHK(hipGraphExecDestroy(p.graph));
+      p.graph = nullptr;
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      try {
+        enqueue_plan_kernels(p, st, false);
+      } catch (...) {
+        (void)hipStreamEndCapture(st, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      HIPCHK(hipStreamEndCapture(st, &g));
+      const hipError_t e = hipGraphInstantiate(&p.graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIPCHK(e);
+      p.graph_layout = p.layout;
+      p.graph_stream = st;
+      p.graph_any_dn = p.any_dn;
+    }
+    HIPCHK(hipGraphLaunch(p.graph, st));
+  } else {
+    enqueue_plan_kernels(p, st, timed);
+  }
+  HIPCHK(hipEventRecord(c.ev1, st));
+}
+
+// After the stream has drained: fold the counters and event times into stats.
+static void collect_plan_stats(BuildPlan& p, hipStream_t st, double t0) {
+  Context& c = ctx();
+  const size_t nl = p.ts.launches.size();
+  std::vector<unsigned long long> cnt(3 * std::max<size_t>(nl, 1));
+  HIPCHK(hipMemcpyAsync(cnt.data(), p.d_cnt.p, cnt.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   float kms = 0.f;
   HIPCHK(hipEventElapsedTime(&kms, c.ev0, c.ev1));
   lowdin_fock_stats stt{};
-  s.launch_ms.assign(nl, 0.f);
-  s.launch_cnt = cnt;
+  const bool timed = p.time_launches && !p.use_graph;
+  p.launch_ms.assign(nl, 0.f);
+  p.launch_cnt = cnt;
   for (size_t k = 0; k < nl; ++k) {
-    const TaskSet::Launch& L = s.intra.launches[k];
-    stt.quartets += cnt[2 * k];
-    stt.prim_quartets += cnt[2 * k + 1];
-    stt.eris += cnt[2 * k] * (uint64_t)(ncart_h(L.la) * ncart_h(L.lb) * ncart_h(L.lc) * ncart_h(L.ld));
-    if (s.time_launches) HIPCHK(hipEventElapsedTime(&s.launch_ms[k], s.launch_ev[2 * k], s.launch_ev[2 * k + 1]));
+    const TaskSet::Launch& L = p.ts.launches[k];
+    const uint64_t ne = (uint64_t)(ncart_h(L.la) * ncart_h(L.lb) * ncart_h(L.lc) * ncart_h(L.ld));
+    stt.quartets += cnt[3 * k];
+    stt.prim_quartets += cnt[3 * k + 1];
+    stt.quartets_fp32 += cnt[3 * k + 2];
+    stt.eris += cnt[3 * k] * ne;
+    stt.eris_fp32 += cnt[3 * k + 2] * ne;
+    if (timed) HIPCHK(hipEventElapsedTime(&p.launch_ms[k], p.launch_ev[2 * k], p.launch_ev[2 * k + 1]));
   }
   stt.kernel_ms = kms;
   stt.total_ms = now_ms() - t0;
-  s.stats = stt;
+  p.stats = stt;
 }
 
 static void compute_2body(Species& s, const double* dens, double* result, double factor) {
@@ -563,12 +786,13 @@ static void compute_2body(Species& s, const double* dens, double* result, double
   ensure_context();
   hipStream_t st = c.stream;
   const size_t nn = (size_t)s.nbasis * s.nbasis;
-  ensure_sig(s, st);
+  BuildPlan& p = own_plan(s);
   s.d_D.upload(dens, nn, st);
   s.d_G.ensure(nn);
-  enqueue_2body(s, s.d_D.p, s.d_G.p, factor, st);
+  enqueue_plan(p, s.d_D.p, s.d_G.p, &factor, st);
   HIPCHK(hipMemcpyAsync(result, s.d_G.p, nn * sizeof(double), hipMemcpyDeviceToHost, st));
-  collect_2body_stats(s, st, t0);
+  collect_plan_stats(p, st, t0);
+  s.stats = p.stats;
 }
 
 static void compute_coupling(Species& a, Species* b, const double* db, double* ra, const double* da, double* rb) {
@@ -599,6 +823,10 @@ static void compute_coupling(Species& a, Species* b, const double* db, double* r
     HIPCHK(hipMemsetAsync(dRb, 0, nb2 * sizeof(double), st));
   }
   HIPCHK(hipEventRecord(c.ev0, st));
+  // all coupling classes on the heavy-kernel stream, joined back to st
+  hipStream_t hs = c.side[0];
+  HIPCHK(hipEventRecord(c.fork, st));
+  HIPCHK(hipStreamWaitEvent(hs, c.fork, 0));
   uint64_t quartets = 0, eris = 0, prims = 0;
   for (const TaskSet::Launch& L : ts->launches) {
     const bool a_bra = L.bra_is_a != 0;
@@ -624,7 +852,7 @@ static void compute_coupling(Species& a, Species* b, const double* db, double* r
       g.rbra = dRb;  g.dket = dDa;
       g.rket = dRa;  g.dbra = dDb;
     }
-    if (!dispatch_cp(L.la, L.lb, L.lc, L.ld, g, st)) throw std::runtime_error("no coupling kernel for class");
+    if (!dispatch_cp(L.la, L.lb, L.lc, L.ld, g, hs)) throw std::runtime_error("no coupling kernel for class");
     for (size_t k = L.off; k < L.off + L.count; ++k) {
       const Task& t = ts->tasks[k];
       const int nq = t.j1 - t.j0;
@@ -635,13 +863,20 @@ static void compute_coupling(Species& a, Species* b, const double* db, double* r
     }
   }
   HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(c.join[0], hs));
+  HIPCHK(hipStreamWaitEvent(st, c.join[0], 0));
   HIPCHK(hipEventRecord(c.ev1, st));
   if (ra) HIPCHK(hipMemcpyAsync(ra, dRa, na2 * sizeof(double), hipMemcpyDeviceToHost, st));
   if (rb) HIPCHK(hipMemcpyAsync(rb, dRb, nb2 * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   float kms = 0.f;
   HIPCHK(hipEventElapsedTime(&kms, c.ev0, c.ev1));
-  lowdin_fock_stats stt{quartets, eris, prims, kms, now_ms() - t0};
+  lowdin_fock_stats stt{};
+  stt.quartets = quartets;
+  stt.eris = eris;
+  stt.prim_quartets = prims;
+  stt.kernel_ms = kms;
+  stt.total_ms = now_ms() - t0;
   a.stats = stt;
 }
 
@@ -697,7 +932,6 @@ void LibintInterface_add_shell(void* lint, double* alpha, double* coeff, double*
     s->generation++;
     s->sig.valid = false;
     s->all.valid = false;
-    s->intra.valid = false;
     lowdin::g_last_error.clear();
   } catch (const std::exception& e) {
     lowdin::g_last_error = std::string("add_shell: ") + e.what();
@@ -733,7 +967,6 @@ void LibintInterface_init_2body_ints(void* lint) {
     if (!s->pairlist_built || list != s->pairlist) {
       s->pairlist = std::move(list);
       s->sig.valid = false;
-      s->intra.valid = false;
     }
     s->pairlist_built = true;
     lowdin::g_last_error.clear();
@@ -770,6 +1003,52 @@ void LibintInterface_compute_coupling_direct(void* lint, void* olint, double* de
 
 const char* lowdin_fock_last_error(void) { return lowdin::g_last_error.c_str(); }
 
+int lowdin_fock_1body(void* lint, int kind, int ncharge, const double* Z, const double* xyz, double* result) {
+  Species* s = static_cast<Species*>(lint);
+  lowdin::Context& c = lowdin::ctx();
+  try {
+    if (kind < 1 || kind > 3) throw std::runtime_error("kind must be 1 (overlap), 2 (kinetic) or 3 (attraction)");
+    std::lock_guard<std::mutex> lk(c.mu);
+    lowdin::ensure_context();
+    hipStream_t st = c.stream;
+    lowdin::ensure_device_basis(*s, st);
+    const int n = s->nbasis, ns = (int)s->shells.size();
+    lowdin::DevBuf<double> dM, dZ, dX;
+    dM.ensure((size_t)n * n);
+    const int nc = kind == 3 ? ncharge : 0;
+    if (nc > 0) {
+      dZ.upload(Z, nc, st);
+      dX.upload(xyz, 3 * (size_t)nc, st);
+    }
+    lowdin::launch_onebody(kind, s->d_shells.p, s->d_pexp.p, s->d_pcoef.p, s->d_norma.p, ns, n, nc, dZ.p, dX.p,
+                           c.boys.p, dM.p, st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(result, dM.p, (size_t)n * n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    lowdin::g_last_error.clear();
+    return 0;
+  } catch (const std::exception& e) {
+    lowdin::g_last_error = std::string("1body: ") + e.what();
+    fprintf(stderr, "lowdin_fock: %s\n", lowdin::g_last_error.c_str());
+    return -1;
+  }
+}
+
+// Libint2Iface.cpp:3482-3503: 1 overlap, 2 kinetic, 3 attraction to the
+// particles added with add_particle (charge z at their centres).
+void LibintInterface_compute_1body_ints(void* lint, int integral_kind, double* result) {
+  Species* s = static_cast<Species*>(lint);
+  std::vector<double> Z, X;
+  for (const auto& p : s->particles) {
+    Z.push_back(p[0]);
+    X.push_back(p[1]);
+    X.push_back(p[2]);
+    X.push_back(p[3]);
+  }
+  if (lowdin_fock_1body(lint, integral_kind, (int)Z.size(), Z.data(), X.data(), result) != 0)
+    lowdin::fill_nan(result, (size_t)s->nbasis * s->nbasis);
+}
+
 int lowdin_fock_compute_2body_device(void* lint, const double* d_dens, double* d_result, double factor,
                                      void* stream, int synchronize) {
   Species* s = static_cast<Species*>(lint);
@@ -779,8 +1058,12 @@ int lowdin_fock_compute_2body_device(void* lint, const double* d_dens, double* d
     const double t0 = lowdin::now_ms();
     lowdin::ensure_context();
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c.stream;
-    lowdin::enqueue_2body(*s, d_dens, d_result, factor, st);
-    if (synchronize) lowdin::collect_2body_stats(*s, st, t0);
+    lowdin::BuildPlan& p = lowdin::own_plan(*s);
+    lowdin::enqueue_plan(p, d_dens, d_result, &factor, st);
+    if (synchronize) {
+      lowdin::collect_plan_stats(p, st, t0);
+      s->stats = p.stats;
+    }
     lowdin::g_last_error.clear();
     return 0;
   } catch (const std::exception& e) {
@@ -791,27 +1074,43 @@ int lowdin_fock_compute_2body_device(void* lint, const double* d_dens, double* d
 }
 
 int lowdin_fock_time_launches(void* lint, int enable) {
-  static_cast<Species*>(lint)->time_launches = enable != 0;
+  lowdin::own_plan(*static_cast<Species*>(lint)).time_launches = enable != 0;
+  return 0;
+}
+
+int lowdin_fock_set_precision(void* lint, int mode, double tau) {
+  Species* s = static_cast<Species*>(lint);
+  if (mode != 0 && mode != 1) {
+    lowdin::g_last_error = "set_precision: mode must be 0 (FP64) or 1 (mixed)";
+    return -1;
+  }
+  s->mixed = mode;
+  s->tau = tau;
+  lowdin::g_last_error.clear();
   return 0;
 }
 
 int lowdin_fock_serialize_launches(void* lint, int enable) {
-  static_cast<Species*>(lint)->serialize = enable != 0;
+  lowdin::own_plan(*static_cast<Species*>(lint)).serialize = enable != 0;
   return 0;
+}
+
+static int plan_profile(const lowdin::BuildPlan& p, int cap, int* classes, float* ms, uint64_t* quartets,
+                        uint64_t* prim_quartets) {
+  const int nl = (int)p.ts.launches.size();
+  for (int k = 0; k < nl && k < cap; ++k) {
+    const lowdin::TaskSet::Launch& L = p.ts.launches[k];
+    classes[k] = L.la * 1000 + L.lb * 100 + L.lc * 10 + L.ld;
+    ms[k] = k < (int)p.launch_ms.size() ? p.launch_ms[k] : 0.f;
+    quartets[k] = 3 * k < (int)p.launch_cnt.size() ? p.launch_cnt[3 * k] : 0;
+    prim_quartets[k] = 3 * k + 1 < (int)p.launch_cnt.size() ? p.launch_cnt[3 * k + 1] : 0;
+  }
+  return nl;
 }
 
 int lowdin_fock_launch_profile(void* lint, int cap, int* classes, float* ms, uint64_t* quartets,
                                uint64_t* prim_quartets) {
-  Species* s = static_cast<Species*>(lint);
-  const int nl = (int)s->intra.launches.size();
-  for (int k = 0; k < nl && k < cap; ++k) {
-    const lowdin::TaskSet::Launch& L = s->intra.launches[k];
-    classes[k] = L.la * 1000 + L.lb * 100 + L.lc * 10 + L.ld;
-    ms[k] = k < (int)s->launch_ms.size() ? s->launch_ms[k] : 0.f;
-    quartets[k] = 2 * k < (int)s->launch_cnt.size() ? s->launch_cnt[2 * k] : 0;
-    prim_quartets[k] = 2 * k + 1 < (int)s->launch_cnt.size() ? s->launch_cnt[2 * k + 1] : 0;
-  }
-  return nl;
+  return plan_profile(lowdin::own_plan(*static_cast<Species*>(lint)), cap, classes, ms, quartets, prim_quartets);
 }
 
 int lowdin_fock_get_stats(void* lint, lowdin_fock_stats* out) {
@@ -867,6 +1166,158 @@ int lowdin_fock_coupling_pair(void* a, void* b, const double* da, const double* 
     lowdin::fill_nan(rb, (size_t)sb->nbasis * sb->nbasis);
     return -1;
   }
+}
+
+// ---------------------------------------------------------------- batches
+struct lowdin_fock_batch_t {
+  lowdin::BuildPlan plan;
+};
+
+void* lowdin_fock_batch_new(int nmol, void* const* lints) {
+  try {
+    if (nmol < 1 || !lints) throw std::runtime_error("a batch needs at least one species handle");
+    auto* b = new lowdin_fock_batch_t();
+    for (int m = 0; m < nmol; ++m) {
+      if (!lints[m]) {
+        delete b;
+        throw std::runtime_error("null species handle");
+      }
+      b->plan.mols.push_back(static_cast<Species*>(lints[m]));
+    }
+    lowdin::g_last_error.clear();
+    return b;
+  } catch (const std::exception& e) {
+    lowdin::g_last_error = std::string("batch_new: ") + e.what();
+    return nullptr;
+  }
+}
+
+void lowdin_fock_batch_del(void* batch) { delete static_cast<lowdin_fock_batch_t*>(batch); }
+
+int64_t lowdin_fock_batch_layout(void* batch, int64_t* offsets) {
+  const lowdin::BuildPlan& p = static_cast<lowdin_fock_batch_t*>(batch)->plan;
+  int64_t o = 0;
+  for (size_t m = 0; m < p.mols.size(); ++m) {
+    if (offsets) offsets[m] = o;
+    o += (int64_t)p.mols[m]->nbasis * p.mols[m]->nbasis;
+  }
+  if (offsets) offsets[p.mols.size()] = o;
+  return o;
+}
+
+int lowdin_fock_batch_compute_2body_device(void* batch, const double* d_dens, double* d_result,
+                                           const double* factors, void* stream, int synchronize) {
+  lowdin::BuildPlan& p = static_cast<lowdin_fock_batch_t*>(batch)->plan;
+  lowdin::Context& c = lowdin::ctx();
+  try {
+    std::lock_guard<std::mutex> lk(c.mu);
+    const double t0 = lowdin::now_ms();
+    lowdin::ensure_context();
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c.stream;
+    lowdin::enqueue_plan(p, d_dens, d_result, factors, st);
+    if (synchronize) lowdin::collect_plan_stats(p, st, t0);
+    lowdin::g_last_error.clear();
+    return 0;
+  } catch (const std::exception& e) {
+    lowdin::g_last_error = std::string("batch_compute_2body_device: ") + e.what();
+    fprintf(stderr, "lowdin_fock: %s\n", lowdin::g_last_error.c_str());
+    return -1;
+  }
+}
+
+int lowdin_fock_batch_options(void* batch, int time_launches, int serialize, int use_graph) {
+  lowdin::BuildPlan& p = static_cast<lowdin_fock_batch_t*>(batch)->plan;
+  std::lock_guard<std::mutex> lk(lowdin::ctx().mu);
+  p.time_launches = time_launches != 0;
+  p.serialize = serialize != 0;
+  p.use_graph = use_graph != 0;
+  return 0;
+}
+
+int lowdin_fock_batch_get_stats(void* batch, lowdin_fock_stats* out) {
+  if (!batch || !out) return -1;
+  *out = static_cast<lowdin_fock_batch_t*>(batch)->plan.stats;
+  return 0;
+}
+
+int lowdin_fock_batch_launch_profile(void* batch, int cap, int* classes, float* ms, uint64_t* quartets,
+                                     uint64_t* prim_quartets) {
+  return plan_profile(static_cast<lowdin_fock_batch_t*>(batch)->plan, cap, classes, ms, quartets, prim_quartets);
+}
+
+// Per class launch: try each candidate workgroup target on serialised,
+// event-timed builds and keep the fastest (ties to the smaller target).
+static int plan_autotune(lowdin::BuildPlan& p, const double* d_dens, double* d_result, const double* factors,
+                         hipStream_t stream, int reps) {
+  lowdin::Context& c = lowdin::ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  lowdin::ensure_context();
+  hipStream_t st = stream ? stream : c.stream;
+  static const int cand[] = {512, 1024, 2048, 4096, 8192};
+  const bool tl = p.time_launches, se = p.serialize, ug = p.use_graph;
+  p.time_launches = true;
+  p.serialize = true;
+  p.use_graph = false;
+  double best[lowdin::kNumClassPairs];
+  int pick[lowdin::kNumClassPairs];
+  for (int k = 0; k < lowdin::kNumClassPairs; ++k) {
+    best[k] = 1e300;
+    pick[k] = p.target_wg[k];
+  }
+  try {
+    for (int wg : cand) {
+      for (int k = 0; k < lowdin::kNumClassPairs; ++k) p.target_wg[k] = wg;
+      p.valid = false;
+      std::vector<double> ms;
+      for (int r = 0; r < reps + 1; ++r) {
+        lowdin::enqueue_plan(p, d_dens, d_result, factors, st);
+        lowdin::collect_plan_stats(p, st, lowdin::now_ms());
+        if (r == 0) {
+          ms.assign(p.ts.launches.size(), 0.0);
+          continue;  // warm-up
+        }
+        for (size_t k = 0; k < p.ts.launches.size(); ++k) ms[k] += p.launch_ms[k];
+      }
+      for (size_t k = 0; k < p.ts.launches.size(); ++k) {
+        const int id = lowdin::class_pair_id(p.ts.launches[k].cbra, p.ts.launches[k].cket);
+        if (ms[k] < 0.97 * best[id]) {
+          best[id] = ms[k];
+          pick[id] = wg;
+        }
+      }
+    }
+  } catch (...) {
+    p.time_launches = tl;
+    p.serialize = se;
+    p.use_graph = ug;
+    throw;
+  }
+  for (int k = 0; k < lowdin::kNumClassPairs; ++k) p.target_wg[k] = pick[k];
+  p.valid = false;
+  p.time_launches = tl;
+  p.serialize = se;
+  p.use_graph = ug;
+  return 0;
+}
+
+int lowdin_fock_batch_autotune(void* batch, const double* d_dens, double* d_result, const double* factors,
+                               void* stream, int reps) {
+  try {
+    plan_autotune(static_cast<lowdin_fock_batch_t*>(batch)->plan, d_dens, d_result, factors,
+                  static_cast<hipStream_t>(stream), std::max(reps, 1));
+    lowdin::g_last_error.clear();
+    return 0;
+  } catch (const std::exception& e) {
+    lowdin::g_last_error = std::string("batch_autotune: ") + e.what();
+    fprintf(stderr, "lowdin_fock: %s\n", lowdin::g_last_error.c_str());
+    return -1;
+  }
+}
+
+int lowdin_fock_batch_targets(void* batch, int* targets) {
+  const lowdin::BuildPlan& p = static_cast<lowdin_fock_batch_t*>(batch)->plan;
+  for (int k = 0; k < lowdin::kNumClassPairs; ++k) targets[k] = p.target_wg[k];
+  return lowdin::kNumClassPairs;
 }
 
 }  // extern "C"

@@ -18,9 +18,6 @@ void set_unsupported(const char* name);
 
 extern "C" {
 
-void LibintInterface_compute_1body_ints(void*, int, double*) {
-  LOWDIN_UNSUPPORTED("LibintInterface_compute_1body_ints");
-}
 void LibintInterface_compute_2body_directIT(void*, double*, double*, double*, int*) {
   LOWDIN_UNSUPPORTED("LibintInterface_compute_2body_directIT");
 }

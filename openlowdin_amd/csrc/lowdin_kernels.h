@@ -8,24 +8,46 @@
 
 namespace lowdin {
 
-struct JKArgs {
-  const Task* tasks;
-  int ntasks;
-  const PairInfo* bra;
-  const PairInfo* ket;
-  const double* prim;      // PrimPair records of this species
+// One molecule (species) of a batched intra-species build.
+struct MolDesc {
+  const PairInfo* pairs;   // significant pairs, grouped by pair class
+  const double* prim;      // PrimPair records
+  const float* primf;      // the same records in FP32 (mixed precision)
   const double* D;         // n x n density
-  const double* Dn;        // nshell x nshell block infinity norms
+  double* Dn;              // nshell x nshell block infinity norms (dnorm_batch writes them)
+  const ShellInfo* shells;
   const double* norma;
-  const double* boys;
   double* Graw;            // n x n raw accumulator (symmetrised afterwards)
-  unsigned int* counts;     // per task: computed quartets, primitive quartets (optional)
+  int off[8];              // pair-class offsets into pairs
   int n;
   int nshell;
   int screen;              // 1: Dnorm*Q*Q < eps screen (electrons)
-  int same_class;          // bra and ket lists are the same class list
-  double eps;
+  int mixed;               // 1: quartets with Dnorm*Q*Q < tau in FP32
   double half_factor;      // 0.5 * exchange factor
+  double tau;
+};
+
+// A class launch over the tasks of one or many molecules; Task.flags is the
+// task's molecule index into mols.
+struct JKArgs {
+  const Task* tasks;
+  int ntasks;
+  const MolDesc* mols;
+  const double* boys;
+  const float* boysf;      // FP32 Boys table
+  unsigned int* counts;    // per task: quartets, primitive quartets, FP32 quartets (optional)
+  int cbra, cket;          // pair classes of the bra and ket lists
+  int same_class;          // bra and ket lists are the same class list
+  int lds_n;               // largest n of the launch (LDS strip sizing)
+  double eps;
+};
+
+// Per molecule of a batched symmetrisation.
+struct SymDesc {
+  const double* Graw;
+  double* G;
+  int n;
+  int pad;
 };
 
 struct CouplingArgs {
@@ -95,7 +117,13 @@ __host__ __device__ inline void overlap_block(const ShellInfo& A, const ShellInf
 void launch_pair_norms(const ShellInfo* sh, const double* pexp, const double* pcoef, int nsh, double* norms,
                        hipStream_t s);
 void launch_dnorm(const double* D, int n, const ShellInfo* sh, int nsh, double* Dn, hipStream_t s);
+void launch_onebody(int kind, const ShellInfo* sh, const double* pexp, const double* pcoef, const double* norma,
+                    int nsh, int n, int ncharge, const double* Z, const double* Cxyz, const double* boys, double* M,
+                    hipStream_t s);
 void launch_symmetrize(const double* Graw, int n, double scale, double* G, hipStream_t s);
-void launch_reduce_counts(const unsigned int* counts, int ntasks, unsigned long long* out, hipStream_t s);
+void launch_symmetrize_batch(const SymDesc* d, int nmol, int nmax, hipStream_t s);
+void launch_fill_zero(double* p, size_t count, hipStream_t s);
+void launch_dnorm_batch(const MolDesc* mols, int nmol, int nshmax, hipStream_t s);
+void launch_reduce_counts(const unsigned int* counts, int ntasks, unsigned long long* out, hipStream_t s);  // 3 per task
 
 }  // namespace lowdin

@@ -36,7 +36,8 @@ class LowdinFockError(RuntimeError):
 
 class Stats(C.Structure):
     _fields_ = [("quartets", C.c_uint64), ("eris", C.c_uint64), ("prim_quartets", C.c_uint64),
-                ("kernel_ms", C.c_double), ("total_ms", C.c_double)]
+                ("kernel_ms", C.c_double), ("total_ms", C.c_double), ("quartets_fp32", C.c_uint64),
+                ("eris_fp32", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -72,9 +73,32 @@ def lib():
         L.lowdin_fock_compute_2body_device.restype = C.c_int
         L.lowdin_fock_time_launches.argtypes = [vp, C.c_int]
         L.lowdin_fock_serialize_launches.argtypes = [vp, C.c_int]
+        L.lowdin_fock_1body.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp]
+        L.lowdin_fock_1body.restype = C.c_int
+        L.LibintInterface_compute_1body_ints.argtypes = [vp, C.c_int, dp]
+        L.lowdin_fock_set_precision.argtypes = [vp, C.c_int, C.c_double]
+        L.lowdin_fock_set_precision.restype = C.c_int
         L.lowdin_fock_launch_profile.argtypes = [vp, C.c_int, ip, C.POINTER(C.c_float),
                                                  C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.lowdin_fock_launch_profile.restype = C.c_int
+        L.lowdin_fock_batch_new.argtypes = [C.c_int, C.POINTER(vp)]
+        L.lowdin_fock_batch_new.restype = vp
+        L.lowdin_fock_batch_del.argtypes = [vp]
+        L.lowdin_fock_batch_layout.argtypes = [vp, C.POINTER(C.c_int64)]
+        L.lowdin_fock_batch_layout.restype = C.c_int64
+        L.lowdin_fock_batch_compute_2body_device.argtypes = [vp, vp, vp, dp, vp, C.c_int]
+        L.lowdin_fock_batch_compute_2body_device.restype = C.c_int
+        L.lowdin_fock_batch_options.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.lowdin_fock_batch_options.restype = C.c_int
+        L.lowdin_fock_batch_get_stats.argtypes = [vp, C.POINTER(Stats)]
+        L.lowdin_fock_batch_get_stats.restype = C.c_int
+        L.lowdin_fock_batch_launch_profile.argtypes = [vp, C.c_int, ip, C.POINTER(C.c_float),
+                                                       C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        L.lowdin_fock_batch_launch_profile.restype = C.c_int
+        L.lowdin_fock_batch_autotune.argtypes = [vp, vp, vp, dp, vp, C.c_int]
+        L.lowdin_fock_batch_autotune.restype = C.c_int
+        L.lowdin_fock_batch_targets.argtypes = [vp, ip]
+        L.lowdin_fock_batch_targets.restype = C.c_int
         _LIB = L
     return _LIB
 
@@ -178,6 +202,25 @@ class Libint2Interface:
 
     def time_launches(self, enable: bool = True):
         lib().lowdin_fock_time_launches(self.this, 1 if enable else 0)
+
+    def one_body(self, kind: int, charges=()) -> np.ndarray:
+        """kind 1 overlap, 2 kinetic (-1/2 nabla^2), 3 attraction to point
+        charges [(Z, xyz bohr)]: sum_C -Z_C/|r - C|.  Normalised, n x n."""
+        Z = np.ascontiguousarray([z for z, _ in charges] or [0.0], dtype=np.float64)
+        X = np.ascontiguousarray([x for _, x in charges] or [[0.0, 0.0, 0.0]], dtype=np.float64).reshape(-1)
+        M = np.zeros((self.nbasis, self.nbasis))
+        nc = len(charges) if kind == 3 else 0
+        if lib().lowdin_fock_1body(self.this, int(kind), nc, _dptr(Z), _dptr(X), _dptr(M)) != 0:
+            _check("1body")
+        return M
+
+    def set_precision(self, mode: str = "fp64", tau: float = 0.0):
+        """'fp64' (reference parity) or 'mixed': quartets with
+        Dnorm_abcd * Qn_ab * Qn_cd < tau (normalised Schwarz bounds times the
+        largest |D| of the quartet's density blocks) are evaluated in FP32."""
+        m = {"fp64": 0, "mixed": 1}[mode]
+        if lib().lowdin_fock_set_precision(self.this, m, float(tau)) != 0:
+            _check("set_precision")
 
     def serialize_launches(self, enable: bool = True):
         lib().lowdin_fock_serialize_launches(self.this, 1 if enable else 0)

@@ -293,3 +293,12 @@ def water_cluster_batch(nmolecules: int = 256, seed: int = BATCH_SEED, basis: st
     rng = np.random.default_rng(seed)
     sizes = rng.integers(1, 9, size=nmolecules)
     return [water_cluster(int(n), rng, basis) for n in sizes]
+
+
+def load_water_batch(path: str, basis: str = "6-31G**"):
+    """The batched workload from its committed geometry fixture (JSON,
+    Angstrom): one MolecularSystem per cluster."""
+    import json
+    with open(path) as fh:
+        data = json.load(fh)
+    return [build_system([(sym, tuple(xyz)) for sym, xyz in g], basis) for g in data["clusters"]]

@@ -70,18 +70,36 @@ class WaveFunction:
         return C
 
 
+def one_body_matrices(system, objs):
+    """H-core and overlap of every species on the GPU:
+    H_i = T_i / m_i - q_i sum_C Z_C (-1/|r - C|) over the point charges."""
+    hcore, overlap = [], []
+    for k, sp in enumerate(system.species):
+        h = LI._instance(objs, system, k + 1)
+        S = h.one_body(1)
+        T = h.one_body(2)
+        V = h.one_body(3, system.charges)
+        hcore.append(T / sp.mass - sp.charge * V)
+        overlap.append(S)
+    return hcore, overlap
+
+
 def _sym_orth(S, thresh=1e-8):
     w, U = np.linalg.eigh(S)
     keep = w > thresh
     return U[:, keep] / np.sqrt(w[keep])
 
 
-def run_scf(system, hcore: List[np.ndarray], overlap: List[np.ndarray], max_iter: int = 300,
+def run_scf(system, hcore: Optional[List[np.ndarray]] = None, overlap: Optional[List[np.ndarray]] = None,
+            max_iter: int = 300,
             e_tol: float = 1e-11, d_tol: float = 1e-9, objs=None, build_for_one_particle: bool = False,
             verbose: bool = False):
     """Multi-species HF with GPU Fock builds.  hcore/overlap per species
-    (normalised Cartesian basis).  Returns (energy, wavefunctions)."""
+    (normalised Cartesian basis; computed on the GPU when not given).
+    Returns (energy, wavefunctions)."""
     objs = {} if objs is None else objs
+    if hcore is None or overlap is None:
+        hcore, overlap = one_body_matrices(system, objs)
     wfs = [WaveFunction(system, k, objs) for k in range(len(system.species))]
     X = [_sym_orth(S) for S in overlap]
 

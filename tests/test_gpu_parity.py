@@ -142,3 +142,40 @@ def test_linearity_large_alkane():
     assert np.max(np.abs(G12 - G1 - G2)) < 1e-9
     npairs = len(h.shellpair_list())
     assert st["quartets"] <= npairs * (npairs + 1) // 2
+
+
+def test_mixed_precision_error_and_fp64_identity():
+    """Mixed precision: quartets with Dnorm*Q*Q < tau in FP32.  tau = 0 must
+    reproduce the FP64 build exactly.  For tau in 1e-4..1e-7 the error of the
+    two-electron energy 1/2 Tr(D G) and of G must shrink with tau (bounded by
+    0.1 tau and 0.5 tau), stay below 1e-6 Eh from tau = 1e-6 down, and FP32
+    must carry a sizeable share of the quartets (butane / cc-pVDZ)."""
+    LI = _li()
+    sp = S.alkane(4, "cc-pVDZ").species[0]
+    rng = np.random.default_rng(29)
+    D = _density(sp.nbasis, sp.n_occupied, rng)
+    _case("mixed precision: butane cc-pVDZ (n=%d)" % sp.nbasis)
+    h = LI.Libint2Interface(sp, 1)
+    G64 = h.compute_2body_intraspecies_direct(D, -0.5)
+    h.set_precision("mixed", 0.0)
+    G0 = h.compute_2body_intraspecies_direct(D, -0.5)
+    assert h.stats()["quartets_fp32"] == 0
+    assert np.max(np.abs(G0 - G64)) < 1e-13
+    rows = []
+    for tau in (1e-4, 1e-5, 1e-6, 1e-7):
+        h.set_precision("mixed", tau)
+        Gm = h.compute_2body_intraspecies_direct(D, -0.5)
+        st = h.stats()
+        dE = 0.5 * float(np.sum(D * (Gm - G64)))
+        dG = float(np.max(np.abs(Gm - G64)))
+        print("   tau %.0e: fp32 quartets %d / %d, max|dG| %.2e, dE %.2e" %
+              (tau, st["quartets_fp32"], st["quartets"], dG, dE), flush=True)
+        rows.append((tau, st["quartets_fp32"], st["quartets"], dE, dG))
+    for tau, q32, q, dE, dG in rows:
+        assert q32 > 0
+        assert abs(dE) < 0.1 * tau
+        assert dG < 0.5 * tau
+        if tau <= 1e-6:
+            assert abs(dE) < 1e-6
+    assert rows[0][1] > 0.3 * rows[0][2]
+    h.set_precision("fp64")
