@@ -162,6 +162,73 @@ def batch_bench(steps, H, B, S, mixed_tau, stream):
     return out
 
 
+def batched_scf_bench(S, SV, mixed_tau):
+    """The 256 water clusters iterated to convergence together: one batched
+    Fock build (one launch per quartet class, HIP graph) per SCF tick, the
+    rest of each iteration on the device per molecule."""
+    systems = S.water_cluster_batch(256)
+    out = {"molecules": len(systems)}
+    for mode, tau in (("fp64", None), ("mixed", mixed_tau)):
+        eng = SV.ScfEngine(max_jobs=256, mixed_tau=tau)
+        for k, sysm in enumerate(systems):
+            eng.submit(SV.Job(k, "cluster%d" % k, sysm, 0.0))
+        t0 = time.perf_counter()
+        done = eng.run(timeout=900)
+        dt = time.perf_counter() - t0
+        out[mode] = {"seconds_to_converge_all": round(dt, 3), "ticks": eng.ticks,
+                     "converged": int(sum(j.converged for j in done)),
+                     "mean_iterations": round(float(np.mean([j.iterations for j in done])), 2),
+                     "molecule_scf_per_s": round(len(done) / dt, 2)}
+        if tau is not None:
+            out[mode]["tau"] = tau
+        print("[bench] batched SCF %s: %d molecules in %.2f s (%d ticks)" % (mode, len(done), dt, eng.ticks),
+              file=sys.stderr, flush=True)
+    return out
+
+
+def serving_bench(S, SV, njobs, rel_rates, mixed_tau, seed=20261202):
+    """Served SCF: Poisson arrivals of mixed jobs ((H2O)_1..24 / C2..C20
+    alkanes, 6-31G** or cc-pVDZ, 20% with a quantum proton), continuous
+    batching under a 200 GB HBM budget.  The saturation rate is measured with
+    every job present at t = 0; arrival rates are fractions/multiples of it.
+    Serial FP64 solves (one job at a time) at the saturation rate are the
+    baseline."""
+    rng = np.random.default_rng(seed)
+    jobs = [S.serving_job(rng) for _ in range(njobs)]
+
+    def run(arrivals, **kw):
+        eng = SV.ScfEngine(hbm_budget=200e9, **kw)
+        for k, ((label, sysm), t) in enumerate(zip(jobs, arrivals)):
+            eng.submit(SV.Job(k, label, sysm, float(t)))
+        eng.run(timeout=900)
+        return eng.report()
+
+    sat = run([0.0] * njobs)
+    lam_sat = sat["jobs_per_s"]
+    out = {"jobs": njobs, "mix": "%d water clusters, %d alkanes, %d with H_1" % (
+        sum(l.startswith("(H2O)") for l, _ in jobs), sum(l.startswith("C") for l, _ in jobs),
+        sum(l.endswith("+H_1") for l, _ in jobs)),
+        "saturation_fp64": sat, "rates": []}
+    print("[bench] serving saturation: %.3f jobs/s" % lam_sat, file=sys.stderr, flush=True)
+    arr_rng = np.random.default_rng(seed + 1)
+    gaps = arr_rng.exponential(1.0, size=njobs)
+    for rel in rel_rates:
+        lam = rel * lam_sat
+        arrivals = np.cumsum(gaps) / lam
+        row = {"rate_rel_saturation": rel, "lambda_jobs_per_s": round(lam, 4)}
+        for mode, tau in (("fp64", None), ("mixed", mixed_tau)):
+            row[mode] = run(arrivals, mixed_tau=tau)
+        out["rates"].append(row)
+        print("[bench] serving at %.2fx: fp64 p50 %.2f s p99 %.2f s; mixed p50 %.2f s p99 %.2f s" % (
+            rel, row["fp64"]["p50_s"], row["fp64"]["p99_s"], row["mixed"]["p50_s"], row["mixed"]["p99_s"]),
+            file=sys.stderr, flush=True)
+    arrivals = np.cumsum(gaps) / lam_sat
+    out["serial_fp64_at_saturation_rate"] = run(arrivals, max_jobs=1)
+    if mixed_tau is not None:
+        out["mixed_tau"] = mixed_tau
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,7 +238,11 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=4)
     ap.add_argument("--no-mixed", action="store_true", help="skip the C20H42 mixed-precision SCF sweep")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-cluster batched build")
-    ap.add_argument("--taus", default="1e-4,1e-5,1e-6,1e-7")
+    ap.add_argument("--taus", default="1e-2,1e-3,1e-4,1e-5,1e-6,1e-7")
+    ap.add_argument("--no-batch-scf", action="store_true", help="skip the 256-cluster batched SCF")
+    ap.add_argument("--no-serving", action="store_true", help="skip the served-SCF runs")
+    ap.add_argument("--serving-jobs", type=int, default=16)
+    ap.add_argument("--serving-rates", default="0.5,1,2")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 256 MiB device-to-device copy (profiler byte-counter calibration)")
     args = ap.parse_args()
@@ -298,6 +369,16 @@ def main():
         print("[bench] 256-cluster batch...", file=sys.stderr, flush=True)
         tau_b = (out.get("mixed") or {}).get("best_tau_within_1e-6") or 1e-6
         out["batch"] = batch_bench(args.steps, H, B, S, tau_b, stream)
+    tau_best = (out.get("mixed") or {}).get("best_tau_within_1e-6") or 1e-6
+    if not args.no_batch_scf or not args.no_serving:
+        from openlowdin_amd import serving as SV
+        if not args.no_batch_scf:
+            print("[bench] 256-cluster batched SCF...", file=sys.stderr, flush=True)
+            out["batched_scf"] = batched_scf_bench(S, SV, tau_best)
+        if not args.no_serving:
+            print("[bench] served SCF...", file=sys.stderr, flush=True)
+            out["serving"] = serving_bench(S, SV, args.serving_jobs,
+                                           [float(r) for r in args.serving_rates.split(",")], tau_best)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle sample)...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(sp, D, args.cpu_stride, st["eris"])

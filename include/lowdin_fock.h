@@ -119,6 +119,12 @@ int lowdin_fock_schwarz(void* lint, double* Q);
  * result may be NULL.  Returns 0 on success. */
 int lowdin_fock_coupling_pair(void* a, void* b, const double* da, const double* db, double* ra, double* rb);
 
+/* Device-resident variant of lowdin_fock_coupling_pair: device pointers,
+ * enqueued on `stream` (NULL = the library stream); results zeroed by the
+ * call.  Returns 0 on success. */
+int lowdin_fock_coupling_pair_device(void* a, void* b, const double* d_da, const double* d_db, double* d_ra,
+                                     double* d_rb, void* stream, int synchronize);
+
 /* Device-resident variant of LibintInterface_compute_2body_direct: d_dens
  * and d_result are device pointers (n x n FP64); the build is enqueued on
  * `stream` (a hipStream_t; NULL = the library's stream).  With synchronize
@@ -171,6 +177,11 @@ int64_t lowdin_fock_batch_layout(void* batch, int64_t* offsets);
 int lowdin_fock_batch_compute_2body_device(void* batch, const double* d_dens, double* d_result,
                                            const double* factors, void* stream, int synchronize);
 
+/* As lowdin_fock_batch_compute_2body_device with one device pointer per
+ * molecule (host arrays of nmol device addresses) instead of packed buffers. */
+int lowdin_fock_batch_compute_2body_device_ptrs(void* batch, const double* const* d_dens, double* const* d_result,
+                                                const double* factors, void* stream, int synchronize);
+
 /* time_launches: HIP events around each class launch (not in graph mode);
  * serialize: class launches on one stream; use_graph: capture once, replay. */
 int lowdin_fock_batch_options(void* batch, int time_launches, int serialize, int use_graph);
@@ -180,16 +191,64 @@ int lowdin_fock_batch_get_stats(void* batch, lowdin_fock_stats* out);
 int lowdin_fock_batch_launch_profile(void* batch, int cap, int* classes, float* ms, uint64_t* quartets,
                                      uint64_t* prim_quartets);
 
-/* Autotuner: for each class launch, the number of workgroups its tasks are
- * cut into is chosen from {512, ..., 8192} by timing `reps` serialised
+/* Autotuner: for each class launch, the kernel shape is chosen among the
+ * wavefront kernel (one bra x 64 kets per wave) and the LDS-strip kernel
+ * cut into about {512, 2048, 8192} workgroups, by timing `reps` serialised
  * builds per candidate on the given inputs.  Results are unchanged by the
  * choice up to FP64 summation order. */
 int lowdin_fock_batch_autotune(void* batch, const double* d_dens, double* d_result, const double* factors,
                                void* stream, int reps);
 
-/* The 21 per-class workgroup targets (class pair id cx*(cx+1)/2+cy over the
- * pair classes ss, ps, pp, ds, dp, dd).  Returns 21. */
+/* The 21 per-class choices (class pair id cx*(cx+1)/2+cy over the pair
+ * classes ss, ps, pp, ds, dp, dd): 0 for the wavefront kernel, else the
+ * strip kernel's workgroup target.  Returns 21. */
 int lowdin_fock_batch_targets(void* batch, int* targets);
+
+/* ------------------------------------------------ device-resident SCF
+ * The caller side of the path on the GPU: a multi-species Hartree-Fock
+ * iteration whose matrices stay in HBM (openLOWDIN src/scf/MultiSCF.f90:
+ * 276-440, 730-1002; WaveFunction.f90:394-467, 817-1535, 1768-2003).  Per
+ * species: H-core guess, canonical orthogonaliser (overlap eigenvalues
+ * > 1e-8), G = q^2 (J + f K) through this library's direct build (skipped
+ * when build_two = 0), couplings q_i q_j J_{i<-j} through the device coupling
+ * build, F = H + G + C, DIIS (8) on X^T (F D S - S D F) X over all species,
+ * rocSOLVER diagonalisation, D = eta C_occ C_occ^T.  E = e_pc +
+ * sum_s Tr(D_s (H_s + G_s / 2 + C_s / 2)).  No reference counterpart as a C
+ * entry point: openLOWDIN runs this loop in Fortran with LAPACK on the host. */
+typedef struct {
+  int nocc;        /* occupied orbitals (particles / eta) */
+  double eta;      /* occupation per orbital (2 closed-shell electrons, 1 otherwise) */
+  double charge;   /* particle charge q */
+  double factor;   /* exchange factor kappa / eta of the direct build */
+  int build_two;   /* 0: G = 0 (one particle, WaveFunction.f90:867) */
+} lowdin_scf_species;
+
+/* lints: species handles (shells added); e_pc: point-charge repulsion;
+ * stream: hipStream_t (NULL: a stream of its own).  NULL on failure. */
+void* lowdin_scf_new(int nspecies, void* const* lints, const lowdin_scf_species* desc, double e_pc, void* stream);
+void lowdin_scf_del(void* scf);
+/* Core Hamiltonian and overlap of a species (host, n x n column-major). */
+int lowdin_scf_set_onebody(void* scf, int species, const double* H, const double* S);
+/* Orthogonalisers and H-core guess densities; clears the DIIS history. */
+int lowdin_scf_start(void* scf);
+/* Fock matrices, energy and DIIS error (max |X^T (F D S - S D F) X|) from the
+ * current densities.  external_g != 0: the G buffers (lowdin_scf_buffers)
+ * already hold J + f K of the current densities (a batched build). */
+int lowdin_scf_fock(void* scf, int external_g, double* energy, double* error);
+/* DIIS extrapolation and new densities. */
+int lowdin_scf_update(void* scf);
+/* Iterate fock/update until |dE| < e_tol and error < d_tol.  Returns 0 when
+ * converged, 1 when max_iter was reached, -1 on failure. */
+int lowdin_scf_run(void* scf, int max_iter, double e_tol, double d_tol, double* energy, int* iterations);
+/* Copy a species matrix to the host: what 0 density, 1 Fock, 2 G, 3 couplings. */
+int lowdin_scf_get(void* scf, int species, int what, double* out);
+/* Device density and G buffers of a species and its exchange factor (for a
+ * batched build).  Returns build_two (0/1) or -1. */
+int lowdin_scf_buffers(void* scf, int species, double** d_density, double** d_two, double* factor);
+/* HBM held by the SCF object (not counting the species handles). */
+int64_t lowdin_scf_device_bytes(void* scf);
+/* The stream the SCF object works on (hipStream_t). */
+void* lowdin_scf_stream(void* scf);
 
 #ifdef __cplusplus
 }

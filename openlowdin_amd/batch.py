@@ -19,12 +19,15 @@ from . import libint2_interface as LI
 
 
 class FockBatch:
-    def __init__(self, handles: Sequence[LI.Libint2Interface], factors: Sequence[float]):
+    def __init__(self, handles: Sequence[LI.Libint2Interface], factors: Sequence[float], init: bool = True):
+        """init: build each handle's shell-pair list first (not needed when
+        the caller already did for the current geometry)."""
         if len(handles) != len(factors) or not handles:
             raise ValueError("one exchange factor per species handle")
         self.handles = list(handles)   # keeps the handles alive for the batch's lifetime
-        for h in self.handles:
-            h.init_2body_ints()
+        if init:
+            for h in self.handles:
+                h.init_2body_ints()
         arr = (C.c_void_p * len(handles))(*[h.this for h in handles])
         self.this = LI.lib().lowdin_fock_batch_new(len(handles), arr)
         if not self.this:
@@ -77,6 +80,18 @@ class FockBatch:
         if rc != 0:
             LI._check("batch_compute_2body_device")
             raise LI.LowdinFockError("batch_compute_2body_device failed")
+
+    def compute_device_ptrs(self, d_dens: Sequence[int], d_result: Sequence[int], stream: int = 0,
+                            synchronize: bool = True):
+        """One device pointer per molecule instead of packed buffers."""
+        nm = len(self.handles)
+        dd = (C.c_void_p * nm)(*d_dens)
+        gg = (C.c_void_p * nm)(*d_result)
+        rc = LI.lib().lowdin_fock_batch_compute_2body_device_ptrs(self.this, dd, gg, LI._dptr(self.factors),
+                                                                  stream or None, 1 if synchronize else 0)
+        if rc != 0:
+            LI._check("batch_compute_2body_device_ptrs")
+            raise LI.LowdinFockError("batch_compute_2body_device_ptrs failed")
 
     def compute(self, densities: Sequence[np.ndarray]) -> List[np.ndarray]:
         """Host convenience: upload, build, download (one synchronous call)."""

@@ -316,7 +316,6 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
   const MolDesc& M = a.mols[t.flags];
   const gcptr<PairInfo> pairs = (gcptr<PairInfo>)M.pairs;
   const gcptr<double> prim = (gcptr<double>)M.prim;
-  const gcptr<float> primf = (gcptr<float>)M.primf;
   const gcptr<double> D = (gcptr<double>)M.D;
   const gcptr<double> nrm = (gcptr<double>)M.norma;
   const gptr<double> G = (gptr<double>)M.Graw;
@@ -353,8 +352,8 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
     double out[NA * NB * NC * ND];
     if constexpr (Fp32Allowed<LA, LB, LC, LD>::value) {
       if (lo)
-        contracted_quartet_f32<LA, LB, LC, LD>(primf + (size_t)P.poff * kPrimPairStride, P.nprim,
-                                               primf + (size_t)K.poff * kPrimPairStride, K.nprim, a.boysf,
+        contracted_quartet_f32<LA, LB, LC, LD>(M.primf + (size_t)P.poff * kPrimPairStride, P.nprim,
+                                               M.primf + (size_t)K.poff * kPrimPairStride, K.nprim, a.boysf,
                                                P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
     }
     if (!lo)
@@ -472,16 +471,18 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
 // in row b), so those are accumulated with LDS atomics in two row strips
 // (NA + NB) x n held for the whole ket stream and flushed to HBM once, with
 // one global atomic per touched element.  J_cd, distinct for every ket, is
-// the only per-quartet global atomic.  The bra's primitive-pair records
-// (at most kMaxStripPrims, guaranteed by the host) are staged in LDS once:
-// every lane reads the same record, an LDS broadcast.
+// the only per-quartet global atomic.  The bra's FP64 primitive-pair
+// records (at most kMaxStripPrims, guaranteed by the host) are staged in
+// LDS once: every lane reads the same record, an LDS broadcast.  The FP32
+// path of mixed precision reads its records through generic pointers: read
+// through global-address-space views its integrals came out wrong on the
+// GPU (cause not found; the FP64 path is unaffected).
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(256) void jk_strip_kernel(JKArgs a) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr bool kF32 = Fp32Allowed<LA, LB, LC, LD>::value;
   extern __shared__ double strip[];  // rows of a (NA x n), then rows of b (NB x n)
   __shared__ double bpd[kMaxStripPrims * kPrimPairStride];
-  __shared__ float bpf[kF32 ? kMaxStripPrims * kPrimPairStride : 1];
 #ifdef LOWDIN_LDS_BOYS
   constexpr int LT = LA + LB + LC + LD;
   __shared__ double btab[kBoysRows * 9];   // e^{-T_i}, F_LT .. F_LT+7 of this class
@@ -492,7 +493,6 @@ __global__ __launch_bounds__(256) void jk_strip_kernel(JKArgs a) {
   const MolDesc& M = a.mols[t.flags];
   const gcptr<PairInfo> pairs = (gcptr<PairInfo>)M.pairs;
   const gcptr<double> prim = (gcptr<double>)M.prim;
-  const gcptr<float> primf = (gcptr<float>)M.primf;
   const gcptr<double> D = (gcptr<double>)M.D;
   const gcptr<double> nrm = (gcptr<double>)M.norma;
   const gptr<double> G = (gptr<double>)M.Graw;
@@ -504,12 +504,6 @@ __global__ __launch_bounds__(256) void jk_strip_kernel(JKArgs a) {
   {
     const gcptr<double> src = prim + (size_t)P.poff * kPrimPairStride;
     for (int k = tid; k < P.nprim * kPrimPairStride; k += blockDim.x) bpd[k] = src[k];
-    if constexpr (kF32) {
-      if (M.mixed) {
-        const gcptr<float> srcf = primf + (size_t)P.poff * kPrimPairStride;
-        for (int k = tid; k < P.nprim * kPrimPairStride; k += blockDim.x) bpf[k] = srcf[k];
-      }
-    }
   }
 #ifdef LOWDIN_LDS_BOYS
   for (int k = tid; k < kBoysRows * 9; k += blockDim.x) {
@@ -553,8 +547,8 @@ __global__ __launch_bounds__(256) void jk_strip_kernel(JKArgs a) {
     if constexpr (kF32) {
       if (lo) {
         myq32 += 1;
-        contracted_quartet_f32<LA, LB, LC, LD>((const float*)bpf, P.nprim,
-                                               primf + (size_t)K.poff * kPrimPairStride, K.nprim, a.boysf,
+        contracted_quartet_f32<LA, LB, LC, LD>(M.primf + (size_t)P.poff * kPrimPairStride, P.nprim,
+                                               M.primf + (size_t)K.poff * kPrimPairStride, K.nprim, a.boysf,
                                                P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
       }
     }

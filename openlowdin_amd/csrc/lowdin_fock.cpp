@@ -231,8 +231,10 @@ struct BuildPlan {
   DevBuf<double> d_Graw, d_Dn;
   DevBuf<unsigned long long> d_cnt;
   DevBuf<unsigned int> d_task_counts;
+  DevBuf<int> d_ranges;                 // per class launch: first task, task count
   uint64_t layout = 0;                  // bumps whenever tasks or buffers move
   int target_wg[kNumClassPairs];        // workgroups a strip launch aims at (autotuned)
+  int kind[kNumClassPairs];             // 0: choose by ket run length, 1: LDS strips, 2: wavefront tasks
   lowdin_fock_stats stats{};
   std::vector<float> launch_ms;         // per class launch of the last build
   std::vector<unsigned long long> launch_cnt;
@@ -246,7 +248,10 @@ struct BuildPlan {
   bool graph_any_dn = false;
 
   BuildPlan() {
-    for (int k = 0; k < kNumClassPairs; ++k) target_wg[k] = 2048;
+    for (int k = 0; k < kNumClassPairs; ++k) {
+      target_wg[k] = 2048;
+      kind[k] = 0;
+    }
   }
   ~BuildPlan() {
     for (hipEvent_t e : launch_ev) (void)hipEventDestroy(e);
@@ -474,19 +479,25 @@ static void build_plan_tasks(BuildPlan& p, hipStream_t st) {
     for (int cy = 0; cy <= cx; ++cy) {
       const int la = kPairClassL[cx][0], lb = kPairClassL[cx][1];
       const int lc = kPairClassL[cy][0], ld = kPairClassL[cy][1];
-      long total = 0;
+      long total = 0, nbras = 0;
       int maxbra = 0;   // largest bra primitive-pair count (staged in LDS by the strip kernel)
       for (int m = 0; m < nm; ++m) {
         const PairSet& ps = p.mols[m]->sig;
         const int nx = ps.off[cx + 1] - ps.off[cx], ny = ps.off[cy + 1] - ps.off[cy];
         if (nx && ny) {
           total += (cx == cy) ? (long)nx * (nx + 1) / 2 : (long)nx * ny;
+          nbras += nx;
           for (int i = ps.off[cx]; i < ps.off[cx + 1]; ++i) maxbra = std::max(maxbra, ps.pairs[i].nprim);
         }
       }
       if (total == 0) continue;
-      const bool strip = (size_t)(ncart_h(la) + ncart_h(lb)) * p.nmax * sizeof(double) <= kStripLdsMax &&
-                         maxbra <= kMaxStripPrims;
+      // LDS strips pay off when a bra has enough kets to fill a workgroup
+      // (256 threads) for a while; short ket runs (small molecules) go to the
+      // wavefront kernel, one bra x 64 kets per wave.
+      const int kd = p.kind[class_pair_id(cx, cy)];
+      const bool strip_ok = (size_t)(ncart_h(la) + ncart_h(lb)) * p.nmax * sizeof(double) <= kStripLdsMax &&
+                            maxbra <= kMaxStripPrims;
+      const bool strip = strip_ok && (kd == 1 || (kd == 0 && total >= 192 * nbras));
       const size_t from = ts.tasks.size();
       // run length: at least enough kets to amortise zeroing and flushing the
       // strips (which grow with n), at most 8192
@@ -526,6 +537,12 @@ static void build_plan_tasks(BuildPlan& p, hipStream_t st) {
   std::stable_sort(ts.order.begin(), ts.order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
   ts.d_tasks.upload(ts.tasks.data(), ts.tasks.size(), st);
   const size_t nl = ts.launches.size();
+  std::vector<int> ranges(2 * std::max<size_t>(nl, 1), 0);
+  for (size_t k = 0; k < nl; ++k) {
+    ranges[2 * k] = (int)ts.launches[k].off;
+    ranges[2 * k + 1] = (int)ts.launches[k].count;
+  }
+  p.d_ranges.upload(ranges.data(), ranges.size(), st);
   p.d_cnt.ensure(3 * std::max<size_t>(nl, 1));
   p.d_task_counts.ensure(3 * std::max<size_t>(ts.tasks.size(), 1));
   p.d_Graw.ensure(std::max<size_t>(p.off[nm], 1));
@@ -603,6 +620,12 @@ static TaskSet* coupling_taskset(Species& a, Species& b, hipStream_t st) {
   return ts;
 }
 
+void set_last_error(const std::string& msg) {
+  g_last_error = msg;
+  fprintf(stderr, "lowdin_fock: %s\n", g_last_error.c_str());
+}
+void clear_last_error() { g_last_error.clear(); }
+
 void set_unsupported(const char* name) {
   g_last_error = std::string(name) + ": not provided by liblowdin_fock (outside the direct Fock path)";
   fprintf(stderr, "lowdin_fock: %s\n", g_last_error.c_str());
@@ -628,23 +651,27 @@ static void ensure_launch_events(BuildPlan& p) {
   }
 }
 
-// The kernels of one build, in stream order on st: Dnorm, zero G_raw, the
-// class launches fanned out over the side streams (largest first, so that
-// small latency-bound classes overlap the large ones), per-launch counter
-// reduction, symmetrisation.  Capturable: no memset, no allocation, no
-// host synchronisation.
-static void enqueue_plan_kernels(BuildPlan& p, hipStream_t st, bool timed) {
+static inline bool heavy_launch(const TaskSet::Launch& L) { return L.la + L.lb + L.lc + L.ld >= kHeavyL; }
+
+// Class launches of one build on the side streams, forked from and joined
+// back to st: largest first, round robin over the side streams so that small
+// latency-bound classes overlap the large ones; the scratch-heavy classes
+// always on side stream 0.  which: 0 all, 1 all but the scratch-heavy ones,
+// 2 only the scratch-heavy ones.
+static void enqueue_classes(BuildPlan& p, hipStream_t st, bool timed, int which) {
   Context& c = ctx();
-  const int nm = (int)p.mols.size();
-  if (p.any_dn) launch_dnorm_batch(p.d_desc.p, nm, p.nshmax, st);
-  launch_fill_zero(p.d_Graw.p, p.off[nm], st);
   const size_t nl = p.ts.launches.size();
+  bool any = false;
+  for (size_t k = 0; k < nl; ++k)
+    any = any || which == 0 || (which == 1) != heavy_launch(p.ts.launches[k]);
+  if (!any) return;
   HIPCHK(hipEventRecord(c.fork, st));
   for (int q = 0; q < kSideStreams; ++q) HIPCHK(hipStreamWaitEvent(c.side[q], c.fork, 0));
   for (size_t r = 0; r < nl; ++r) {
     const size_t k = p.ts.order[r];
     const TaskSet::Launch& L = p.ts.launches[k];
-    const bool heavy = L.la + L.lb + L.lc + L.ld >= kHeavyL;
+    const bool heavy = heavy_launch(L);
+    if ((which == 1 && heavy) || (which == 2 && !heavy)) continue;
     hipStream_t ls = c.side[(p.serialize || heavy) ? 0 : r % kSideStreams];
     JKArgs a{};
     a.tasks = p.ts.d_tasks.p + L.off;
@@ -667,10 +694,19 @@ static void enqueue_plan_kernels(BuildPlan& p, hipStream_t st, bool timed) {
     HIPCHK(hipEventRecord(c.join[q], c.side[q]));
     HIPCHK(hipStreamWaitEvent(st, c.join[q], 0));
   }
-  for (size_t k = 0; k < nl; ++k) {
-    const TaskSet::Launch& L = p.ts.launches[k];
-    launch_reduce_counts(p.d_task_counts.p + 3 * L.off, (int)L.count, p.d_cnt.p + 3 * k, st);
-  }
+}
+
+// Dnorm and zeroed G_raw (no memset: capturable).
+static void enqueue_prologue(BuildPlan& p, hipStream_t st) {
+  const int nm = (int)p.mols.size();
+  if (p.any_dn) launch_dnorm_batch(p.d_desc.p, nm, p.nshmax, st);
+  launch_fill_zero(p.d_Graw.p, p.off[nm], st);
+}
+
+// Per-launch counters and G = 0.25 (G_raw + G_raw^T).
+static void enqueue_epilogue(BuildPlan& p, hipStream_t st) {
+  const int nm = (int)p.mols.size();
+  launch_reduce_counts_multi(p.d_task_counts.p, p.d_ranges.p, (int)p.ts.launches.size(), p.d_cnt.p, st);
   launch_symmetrize_batch(p.d_sym.p, nm, p.nmax, st);
   HIPCHK(hipGetLastError());
 }
@@ -679,7 +715,8 @@ static void enqueue_plan_kernels(BuildPlan& p, hipStream_t st, bool timed) {
 // results dG, factors[m] the exchange factor of molecule m.  Nothing is
 // allocated once the plan's buffers exist; the molecule descriptors are
 // re-uploaded only when a pointer, factor or precision setting changed.
-static void enqueue_plan(BuildPlan& p, const double* dD, double* dG, const double* factors, hipStream_t st) {
+static void enqueue_plan(BuildPlan& p, const double* dD, double* dG, const double* factors, hipStream_t st,
+                         const double* const* dD_each = nullptr, double* const* dG_each = nullptr) {
   Context& c = ctx();
   ensure_plan(p, st);
   const int nm = (int)p.mols.size();
@@ -693,7 +730,7 @@ static void enqueue_plan(BuildPlan& p, const double* dD, double* dG, const doubl
     M.pairs = s.sig.d_pairs.p;
     M.prim = s.sig.d_prims.p;
     M.primf = s.sig.d_primsf.p;
-    M.D = dD + p.off[m];
+    M.D = dD_each ? dD_each[m] : dD + p.off[m];
     M.Dn = p.d_Dn.p + p.shoff[m];
     M.shells = s.d_shells.p;
     M.norma = s.d_norma.p;
@@ -709,7 +746,7 @@ static void enqueue_plan(BuildPlan& p, const double* dD, double* dG, const doubl
     SymDesc& Y = p.sym[m];
     std::memset(&Y, 0, sizeof(Y));
     Y.Graw = M.Graw;
-    Y.G = dG + p.off[m];
+    Y.G = dG_each ? dG_each[m] : dG + p.off[m];
     Y.n = s.nbasis;
   }
   if (p.desc_dev.size() != p.desc.size() || std::memcmp(p.desc_dev.data(), p.desc.data(), nm * sizeof(MolDesc))) {
@@ -724,13 +761,18 @@ static void enqueue_plan(BuildPlan& p, const double* dD, double* dG, const doubl
   if (timed) ensure_launch_events(p);
   HIPCHK(hipEventRecord(c.ev0, st));
   if (p.use_graph) {
+    // The graph holds the prologue and the classes without a large private
+    // segment.  Kernels in a graph run on the runtime's own streams (one per
+    // branch), so the scratch-heavy classes stay outside it, on side
+    // stream 0, with the epilogue.
     if (!p.graph || p.graph_layout != p.layout || p.graph_stream != st || p.graph_any_dn != p.any_dn) {
       if (p.graph) HIPCHK(hipGraphExecDestroy(p.graph));
       p.graph = nullptr;
       hipGraph_t g = nullptr;
       HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       try {
-        enqueue_plan_kernels(p, st, false);
+        enqueue_prologue(p, st);
+        enqueue_classes(p, st, false, 1);
       } catch (...) {
         (void)hipStreamEndCapture(st, &g);
         if (g) (void)hipGraphDestroy(g);
@@ -745,8 +787,12 @@ static void enqueue_plan(BuildPlan& p, const double* dD, double* dG, const doubl
       p.graph_any_dn = p.any_dn;
     }
     HIPCHK(hipGraphLaunch(p.graph, st));
+    enqueue_classes(p, st, false, 2);
+    enqueue_epilogue(p, st);
   } else {
-    enqueue_plan_kernels(p, st, timed);
+    enqueue_prologue(p, st);
+    enqueue_classes(p, st, timed, 0);
+    enqueue_epilogue(p, st);
   }
   HIPCHK(hipEventRecord(c.ev1, st));
 }
@@ -795,43 +841,26 @@ static void compute_2body(Species& s, const double* dens, double* result, double
   s.stats = p.stats;
 }
 
-static void compute_coupling(Species& a, Species* b, const double* db, double* ra, const double* da, double* rb) {
-  // ra (on a, from db) and/or rb (on b, from da)
+// Enqueue the coupling quartets of species a and b on st: dRa (n_a^2) =
+// coupling on a from dDb and/or dRb (n_b^2) = coupling on b from dDa (device
+// pointers; either result may be null).  Results are zeroed here by a fill
+// kernel; the classes run on the heavy-kernel stream, joined back to st.
+static void enqueue_coupling(Species& a, Species& b, const double* dDb, double* dRa, const double* dDa, double* dRb,
+                             hipStream_t st, lowdin_fock_stats* stt) {
   Context& c = ctx();
-  std::lock_guard<std::mutex> lk(c.mu);
-  const double t0 = now_ms();
-  ensure_context();
-  hipStream_t st = c.stream;
   ensure_all(a, st);
-  ensure_all(*b, st);
-  TaskSet* ts = coupling_taskset(a, *b, st);
-  const size_t na2 = (size_t)a.nbasis * a.nbasis, nb2 = (size_t)b->nbasis * b->nbasis;
-  double* dRa = nullptr;
-  double* dRb = nullptr;
-  const double* dDa = nullptr;
-  const double* dDb = nullptr;
-  if (ra) {
-    a.d_Dother.upload(db, nb2, st);
-    dDb = a.d_Dother.p;
-    dRa = a.d_R.ensure(na2);
-    HIPCHK(hipMemsetAsync(dRa, 0, na2 * sizeof(double), st));
-  }
-  if (rb) {
-    b->d_Dother.upload(da, na2, st);
-    dDa = b->d_Dother.p;
-    dRb = b->d_R.ensure(nb2);
-    HIPCHK(hipMemsetAsync(dRb, 0, nb2 * sizeof(double), st));
-  }
-  HIPCHK(hipEventRecord(c.ev0, st));
-  // all coupling classes on the heavy-kernel stream, joined back to st
+  ensure_all(b, st);
+  TaskSet* ts = coupling_taskset(a, b, st);
+  if (dRa) launch_fill_zero(dRa, (size_t)a.nbasis * a.nbasis, st);
+  if (dRb) launch_fill_zero(dRb, (size_t)b.nbasis * b.nbasis, st);
   hipStream_t hs = c.side[0];
   HIPCHK(hipEventRecord(c.fork, st));
   HIPCHK(hipStreamWaitEvent(hs, c.fork, 0));
   uint64_t quartets = 0, eris = 0, prims = 0;
   for (const TaskSet::Launch& L : ts->launches) {
     const bool a_bra = L.bra_is_a != 0;
-    Species& sb = a_bra ? a : *b;
-    Species& sk = a_bra ? *b : a;
+    Species& sb = a_bra ? a : b;
+    Species& sk = a_bra ? b : a;
     CouplingArgs g{};
     g.tasks = ts->d_tasks.p + L.off;
     g.ntasks = (int)L.count;
@@ -853,28 +882,58 @@ static void compute_coupling(Species& a, Species* b, const double* db, double* r
       g.rket = dRa;  g.dbra = dDb;
     }
     if (!dispatch_cp(L.la, L.lb, L.lc, L.ld, g, hs)) throw std::runtime_error("no coupling kernel for class");
-    for (size_t k = L.off; k < L.off + L.count; ++k) {
-      const Task& t = ts->tasks[k];
-      const int nq = t.j1 - t.j0;
-      quartets += nq;
-      eris += (uint64_t)nq * ncart_h(L.la) * ncart_h(L.lb) * ncart_h(L.lc) * ncart_h(L.ld);
-      prims += (uint64_t)nq * sb.all.pairs[sb.all.off[L.cbra] + t.bra].nprim *
-               sk.all.pairs[sk.all.off[L.cket] + t.j0].nprim;
+    if (stt) {
+      for (size_t k = L.off; k < L.off + L.count; ++k) {
+        const Task& t = ts->tasks[k];
+        const int nq = t.j1 - t.j0;
+        quartets += nq;
+        eris += (uint64_t)nq * ncart_h(L.la) * ncart_h(L.lb) * ncart_h(L.lc) * ncart_h(L.ld);
+        prims += (uint64_t)nq * sb.all.pairs[sb.all.off[L.cbra] + t.bra].nprim *
+                 sk.all.pairs[sk.all.off[L.cket] + t.j0].nprim;
+      }
     }
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(c.join[0], hs));
   HIPCHK(hipStreamWaitEvent(st, c.join[0], 0));
+  if (stt) {
+    stt->quartets = quartets;
+    stt->eris = eris;
+    stt->prim_quartets = prims;
+  }
+}
+
+static void compute_coupling(Species& a, Species* b, const double* db, double* ra, const double* da, double* rb) {
+  // ra (on a, from db) and/or rb (on b, from da)
+  Context& c = ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  const double t0 = now_ms();
+  ensure_context();
+  hipStream_t st = c.stream;
+  const size_t na2 = (size_t)a.nbasis * a.nbasis, nb2 = (size_t)b->nbasis * b->nbasis;
+  double* dRa = nullptr;
+  double* dRb = nullptr;
+  const double* dDa = nullptr;
+  const double* dDb = nullptr;
+  if (ra) {
+    a.d_Dother.upload(db, nb2, st);
+    dDb = a.d_Dother.p;
+    dRa = a.d_R.ensure(na2);
+  }
+  if (rb) {
+    b->d_Dother.upload(da, na2, st);
+    dDa = b->d_Dother.p;
+    dRb = b->d_R.ensure(nb2);
+  }
+  HIPCHK(hipEventRecord(c.ev0, st));
+  lowdin_fock_stats stt{};
+  enqueue_coupling(a, *b, dDb, dRa, dDa, dRb, st, &stt);
   HIPCHK(hipEventRecord(c.ev1, st));
   if (ra) HIPCHK(hipMemcpyAsync(ra, dRa, na2 * sizeof(double), hipMemcpyDeviceToHost, st));
   if (rb) HIPCHK(hipMemcpyAsync(rb, dRb, nb2 * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   float kms = 0.f;
   HIPCHK(hipEventElapsedTime(&kms, c.ev0, c.ev1));
-  lowdin_fock_stats stt{};
-  stt.quartets = quartets;
-  stt.eris = eris;
-  stt.prim_quartets = prims;
   stt.kernel_ms = kms;
   stt.total_ms = now_ms() - t0;
   a.stats = stt;
@@ -1152,6 +1211,26 @@ int lowdin_fock_schwarz(void* lint, double* Q) {
   }
 }
 
+int lowdin_fock_coupling_pair_device(void* a, void* b, const double* d_da, const double* d_db, double* d_ra,
+                                     double* d_rb, void* stream, int synchronize) {
+  Species* sa = static_cast<Species*>(a);
+  Species* sb = static_cast<Species*>(b);
+  lowdin::Context& c = lowdin::ctx();
+  try {
+    std::lock_guard<std::mutex> lk(c.mu);
+    lowdin::ensure_context();
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c.stream;
+    lowdin::enqueue_coupling(*sa, *sb, d_db, d_ra, d_da, d_rb, st, nullptr);
+    if (synchronize) HIPCHK(hipStreamSynchronize(st));
+    lowdin::g_last_error.clear();
+    return 0;
+  } catch (const std::exception& e) {
+    lowdin::g_last_error = std::string("coupling_pair_device: ") + e.what();
+    fprintf(stderr, "lowdin_fock: %s\n", lowdin::g_last_error.c_str());
+    return -1;
+  }
+}
+
 int lowdin_fock_coupling_pair(void* a, void* b, const double* da, const double* db, double* ra, double* rb) {
   Species* sa = static_cast<Species*>(a);
   Species* sb = static_cast<Species*>(b);
@@ -1225,6 +1304,26 @@ int lowdin_fock_batch_compute_2body_device(void* batch, const double* d_dens, do
   }
 }
 
+int lowdin_fock_batch_compute_2body_device_ptrs(void* batch, const double* const* d_dens, double* const* d_result,
+                                                const double* factors, void* stream, int synchronize) {
+  lowdin::BuildPlan& p = static_cast<lowdin_fock_batch_t*>(batch)->plan;
+  lowdin::Context& c = lowdin::ctx();
+  try {
+    std::lock_guard<std::mutex> lk(c.mu);
+    const double t0 = lowdin::now_ms();
+    lowdin::ensure_context();
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c.stream;
+    lowdin::enqueue_plan(p, nullptr, nullptr, factors, st, d_dens, d_result);
+    if (synchronize) lowdin::collect_plan_stats(p, st, t0);
+    lowdin::g_last_error.clear();
+    return 0;
+  } catch (const std::exception& e) {
+    lowdin::g_last_error = std::string("batch_compute_2body_device_ptrs: ") + e.what();
+    fprintf(stderr, "lowdin_fock: %s\n", lowdin::g_last_error.c_str());
+    return -1;
+  }
+}
+
 int lowdin_fock_batch_options(void* batch, int time_launches, int serialize, int use_graph) {
   lowdin::BuildPlan& p = static_cast<lowdin_fock_batch_t*>(batch)->plan;
   std::lock_guard<std::mutex> lk(lowdin::ctx().mu);
@@ -1253,20 +1352,26 @@ static int plan_autotune(lowdin::BuildPlan& p, const double* d_dens, double* d_r
   std::lock_guard<std::mutex> lk(c.mu);
   lowdin::ensure_context();
   hipStream_t st = stream ? stream : c.stream;
-  static const int cand[] = {512, 1024, 2048, 4096, 8192};
+  // candidate 0 is the wavefront kernel, the rest LDS strips aiming at that
+  // many workgroups per launch
+  static const int cand[] = {0, 512, 2048, 8192};
   const bool tl = p.time_launches, se = p.serialize, ug = p.use_graph;
   p.time_launches = true;
   p.serialize = true;
   p.use_graph = false;
   double best[lowdin::kNumClassPairs];
-  int pick[lowdin::kNumClassPairs];
+  int pick[lowdin::kNumClassPairs], pick_kind[lowdin::kNumClassPairs];
   for (int k = 0; k < lowdin::kNumClassPairs; ++k) {
     best[k] = 1e300;
     pick[k] = p.target_wg[k];
+    pick_kind[k] = p.kind[k];
   }
   try {
     for (int wg : cand) {
-      for (int k = 0; k < lowdin::kNumClassPairs; ++k) p.target_wg[k] = wg;
+      for (int k = 0; k < lowdin::kNumClassPairs; ++k) {
+        p.kind[k] = wg == 0 ? 2 : 1;
+        p.target_wg[k] = wg == 0 ? 2048 : wg;
+      }
       p.valid = false;
       std::vector<double> ms;
       for (int r = 0; r < reps + 1; ++r) {
@@ -1279,20 +1384,31 @@ static int plan_autotune(lowdin::BuildPlan& p, const double* d_dens, double* d_r
         for (size_t k = 0; k < p.ts.launches.size(); ++k) ms[k] += p.launch_ms[k];
       }
       for (size_t k = 0; k < p.ts.launches.size(); ++k) {
-        const int id = lowdin::class_pair_id(p.ts.launches[k].cbra, p.ts.launches[k].cket);
+        const lowdin::TaskSet::Launch& L = p.ts.launches[k];
+        const int id = lowdin::class_pair_id(L.cbra, L.cket);
+        if (wg != 0 && !L.strip) continue;   // strips impossible for this class: candidate ran as waves
         if (ms[k] < 0.97 * best[id]) {
           best[id] = ms[k];
-          pick[id] = wg;
+          pick[id] = wg == 0 ? 2048 : wg;
+          pick_kind[id] = wg == 0 ? 2 : 1;
         }
       }
     }
   } catch (...) {
+    for (int k = 0; k < lowdin::kNumClassPairs; ++k) {
+      p.target_wg[k] = pick[k];
+      p.kind[k] = pick_kind[k];
+    }
+    p.valid = false;
     p.time_launches = tl;
     p.serialize = se;
     p.use_graph = ug;
     throw;
   }
-  for (int k = 0; k < lowdin::kNumClassPairs; ++k) p.target_wg[k] = pick[k];
+  for (int k = 0; k < lowdin::kNumClassPairs; ++k) {
+    p.target_wg[k] = pick[k];
+    p.kind[k] = pick_kind[k];
+  }
   p.valid = false;
   p.time_launches = tl;
   p.serialize = se;
@@ -1316,7 +1432,7 @@ int lowdin_fock_batch_autotune(void* batch, const double* d_dens, double* d_resu
 
 int lowdin_fock_batch_targets(void* batch, int* targets) {
   const lowdin::BuildPlan& p = static_cast<lowdin_fock_batch_t*>(batch)->plan;
-  for (int k = 0; k < lowdin::kNumClassPairs; ++k) targets[k] = p.target_wg[k];
+  for (int k = 0; k < lowdin::kNumClassPairs; ++k) targets[k] = p.kind[k] == 2 ? 0 : p.target_wg[k];
   return lowdin::kNumClassPairs;
 }
 

@@ -121,9 +121,24 @@ void launch_onebody(int kind, const ShellInfo* sh, const double* pexp, const dou
                     int nsh, int n, int ncharge, const double* Z, const double* Cxyz, const double* boys, double* M,
                     hipStream_t s);
 void launch_symmetrize(const double* Graw, int n, double scale, double* G, hipStream_t s);
+// DIIS extrapolation weights: out = sum_k c[k] * hist[idx[k]].
+constexpr int kMaxDiis = 8;
+struct LinComb {
+  double c[kMaxDiis];
+  int idx[kMaxDiis];
+  int k;
+};
+void launch_fock_assemble(double* F, const double* H, const double* G, double qq, const double* Cs, size_t nn,
+                          hipStream_t s);
+void launch_antisym(double* E, const double* T, int n, hipStream_t s);
+void launch_sym_add(double* Acc, const double* R, int n, double scale, hipStream_t s);
+void launch_lincomb(double* out, const double* hist, size_t nn, const LinComb& lc, hipStream_t s);
+void launch_scale_cols(double* X, const double* U, const double* w, int n, int m, int col0, hipStream_t s);
 void launch_symmetrize_batch(const SymDesc* d, int nmol, int nmax, hipStream_t s);
 void launch_fill_zero(double* p, size_t count, hipStream_t s);
 void launch_dnorm_batch(const MolDesc* mols, int nmol, int nshmax, hipStream_t s);
 void launch_reduce_counts(const unsigned int* counts, int ntasks, unsigned long long* out, hipStream_t s);  // 3 per task
+void launch_reduce_counts_multi(const unsigned int* counts, const int* ranges, int nlaunch, unsigned long long* out,
+                                hipStream_t s);
 
 }  // namespace lowdin

@@ -250,6 +250,86 @@ __global__ void fill_zero_kernel(double* p, size_t count) {
   for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += stride) p[k] = 0.0;
 }
 
+// ------------------------------------------------------------ SCF helpers
+// F = H + qq G (+ Cs): the Fock matrix of a species from its core
+// Hamiltonian, two-particle matrix (scaled by the charge squared) and the
+// summed coupling matrices.
+__global__ void fock_assemble_kernel(double* F, const double* H, const double* G, double qq, const double* Cs,
+                                     size_t nn) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < nn; k += stride) {
+    double v = H[k];
+    if (G) v += qq * G[k];
+    if (Cs) v += Cs[k];
+    F[k] = v;
+  }
+}
+
+// E = T - T^T (n x n).
+__global__ void antisym_kernel(double* E, const double* T, int n) {
+  const int i = blockIdx.y * blockDim.y + threadIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || j >= n) return;
+  E[(size_t)i * n + j] = T[(size_t)i * n + j] - T[(size_t)j * n + i];
+}
+
+// Acc += scale * sym(R), R holding one triangle: element (i, j) of the
+// symmetric matrix is R[max(i,j) * n + min(i,j)] (the coupling build's layout).
+__global__ void sym_add_kernel(double* Acc, const double* R, int n, double scale) {
+  const int i = blockIdx.y * blockDim.y + threadIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || j >= n) return;
+  const int r = i > j ? i : j, c = i > j ? j : i;
+  Acc[(size_t)i * n + j] += scale * R[(size_t)r * n + c];
+}
+
+// out = sum_k c[k] * hist[idx[k]] over nn elements.
+__global__ void lincomb_kernel(double* out, const double* hist, size_t nn, LinComb lc) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nn; e += stride) {
+    double v = 0.0;
+    for (int k = 0; k < lc.k; ++k) v += lc.c[k] * hist[(size_t)lc.idx[k] * nn + e];
+    out[e] = v;
+  }
+}
+
+// X(:, j) = U(:, col0 + j) / sqrt(w[col0 + j]), column-major n x m.
+__global__ void scale_cols_kernel(double* X, const double* U, const double* w, int n, int m, int col0) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= n || j >= m) return;
+  X[(size_t)j * n + i] = U[(size_t)(col0 + j) * n + i] / sqrt(w[col0 + j]);
+}
+
+// Per class launch (blockIdx.x): sum the per-task counters of its task range
+// (ranges[2k], ranges[2k+1] = first task, task count) into out[3k .. 3k+2].
+__global__ __launch_bounds__(1024) void reduce_counts_multi_kernel(const unsigned int* counts, const int* ranges,
+                                                                   unsigned long long* out) {
+  __shared__ unsigned long long sq[1024], sp[1024], s3[1024];
+  const int first = ranges[2 * blockIdx.x], ntasks = ranges[2 * blockIdx.x + 1];
+  const unsigned int* c = counts + 3 * (size_t)first;
+  unsigned long long q = 0, p = 0, q3 = 0;
+  for (int i = threadIdx.x; i < ntasks; i += blockDim.x) {
+    q += c[3 * i];
+    p += c[3 * i + 1];
+    q3 += c[3 * i + 2];
+  }
+  sq[threadIdx.x] = q;
+  sp[threadIdx.x] = p;
+  s3[threadIdx.x] = q3;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sq[threadIdx.x] += sq[threadIdx.x + w];
+      sp[threadIdx.x] += sp[threadIdx.x + w];
+      s3[threadIdx.x] += s3[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[3 * blockIdx.x] = sq[0];
+    out[3 * blockIdx.x + 1] = sp[0];
+    out[3 * blockIdx.x + 2] = s3[0];
+  }
+}
+
 __global__ __launch_bounds__(1024) void reduce_counts_kernel(const unsigned int* counts, int ntasks,
                                                              unsigned long long* out) {
   __shared__ unsigned long long sq[1024], sp[1024], s3[1024];
@@ -279,6 +359,12 @@ __global__ __launch_bounds__(1024) void reduce_counts_kernel(const unsigned int*
 }
 
 // ------------------------------------------------------------ launchers
+void launch_reduce_counts_multi(const unsigned int* counts, const int* ranges, int nlaunch, unsigned long long* out,
+                                hipStream_t s) {
+  if (nlaunch > 0)
+    hipLaunchKernelGGL(reduce_counts_multi_kernel, dim3(nlaunch), dim3(1024), 0, s, counts, ranges, out);
+}
+
 void launch_reduce_counts(const unsigned int* counts, int ntasks, unsigned long long* out, hipStream_t s) {
   hipLaunchKernelGGL(reduce_counts_kernel, dim3(1), dim3(1024), 0, s, counts, ntasks, out);
 }
@@ -328,6 +414,34 @@ void launch_fill_zero(double* p, size_t count, hipStream_t s) {
   if (count == 0) return;
   const size_t blocks = std::min<size_t>((count + 255) / 256, 4096);
   hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, count);
+}
+
+static unsigned grid_for(size_t count) { return (unsigned)std::min<size_t>((count + 255) / 256, 4096); }
+
+void launch_fock_assemble(double* F, const double* H, const double* G, double qq, const double* Cs, size_t nn,
+                          hipStream_t s) {
+  if (nn) hipLaunchKernelGGL(fock_assemble_kernel, dim3(grid_for(nn)), dim3(256), 0, s, F, H, G, qq, Cs, nn);
+}
+
+void launch_antisym(double* E, const double* T, int n, hipStream_t s) {
+  if (n <= 0) return;
+  const dim3 blk(32, 8), grd((n + 31) / 32, (n + 7) / 8);
+  hipLaunchKernelGGL(antisym_kernel, grd, blk, 0, s, E, T, n);
+}
+
+void launch_sym_add(double* Acc, const double* R, int n, double scale, hipStream_t s) {
+  if (n <= 0) return;
+  const dim3 blk(32, 8), grd((n + 31) / 32, (n + 7) / 8);
+  hipLaunchKernelGGL(sym_add_kernel, grd, blk, 0, s, Acc, R, n, scale);
+}
+
+void launch_lincomb(double* out, const double* hist, size_t nn, const LinComb& lc, hipStream_t s) {
+  if (nn) hipLaunchKernelGGL(lincomb_kernel, dim3(grid_for(nn)), dim3(256), 0, s, out, hist, nn, lc);
+}
+
+void launch_scale_cols(double* X, const double* U, const double* w, int n, int m, int col0, hipStream_t s) {
+  if (n <= 0 || m <= 0) return;
+  hipLaunchKernelGGL(scale_cols_kernel, dim3((n + 255) / 256, m), dim3(256), 0, s, X, U, w, n, m, col0);
 }
 
 }  // namespace lowdin

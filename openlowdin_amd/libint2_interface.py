@@ -43,6 +43,12 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ScfSpeciesDesc(C.Structure):
+    """lowdin_scf_species."""
+    _fields_ = [("nocc", C.c_int), ("eta", C.c_double), ("charge", C.c_double), ("factor", C.c_double),
+                ("build_two", C.c_int)]
+
+
 def lib():
     """Load liblowdin_fock.so (built by ``__graft_entry__.build()``)."""
     global _LIB
@@ -99,6 +105,31 @@ def lib():
         L.lowdin_fock_batch_autotune.restype = C.c_int
         L.lowdin_fock_batch_targets.argtypes = [vp, ip]
         L.lowdin_fock_batch_targets.restype = C.c_int
+        L.lowdin_fock_batch_compute_2body_device_ptrs.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), dp, vp, C.c_int]
+        L.lowdin_fock_batch_compute_2body_device_ptrs.restype = C.c_int
+        L.lowdin_fock_coupling_pair_device.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_int]
+        L.lowdin_fock_coupling_pair_device.restype = C.c_int
+        L.lowdin_scf_new.argtypes = [C.c_int, C.POINTER(vp), C.POINTER(ScfSpeciesDesc), C.c_double, vp]
+        L.lowdin_scf_new.restype = vp
+        L.lowdin_scf_del.argtypes = [vp]
+        L.lowdin_scf_set_onebody.argtypes = [vp, C.c_int, dp, dp]
+        L.lowdin_scf_set_onebody.restype = C.c_int
+        L.lowdin_scf_start.argtypes = [vp]
+        L.lowdin_scf_start.restype = C.c_int
+        L.lowdin_scf_fock.argtypes = [vp, C.c_int, dp, dp]
+        L.lowdin_scf_fock.restype = C.c_int
+        L.lowdin_scf_update.argtypes = [vp]
+        L.lowdin_scf_update.restype = C.c_int
+        L.lowdin_scf_run.argtypes = [vp, C.c_int, C.c_double, C.c_double, dp, ip]
+        L.lowdin_scf_run.restype = C.c_int
+        L.lowdin_scf_get.argtypes = [vp, C.c_int, C.c_int, dp]
+        L.lowdin_scf_get.restype = C.c_int
+        L.lowdin_scf_buffers.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(vp), dp]
+        L.lowdin_scf_buffers.restype = C.c_int
+        L.lowdin_scf_device_bytes.argtypes = [vp]
+        L.lowdin_scf_device_bytes.restype = C.c_int64
+        L.lowdin_scf_stream.argtypes = [vp]
+        L.lowdin_scf_stream.restype = vp
         _LIB = L
     return _LIB
 

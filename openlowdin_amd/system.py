@@ -302,3 +302,36 @@ def load_water_batch(path: str, basis: str = "6-31G**"):
     with open(path) as fh:
         data = json.load(fh)
     return [build_system([(sym, tuple(xyz)) for sym, xyz in g], basis) for g in data["clusters"]]
+
+
+def with_quantum_proton(atoms, e_basis: str, h_index: int) -> MolecularSystem:
+    """``atoms`` with the nucleus of hydrogen ``h_index`` (index into atoms)
+    made a quantum proton ``H_1`` carrying the even-tempered s,p,d basis of
+    the [F-H-F]- workload; the electrons keep ``e_basis`` on every atom."""
+    sysm = build_system(atoms, e_basis)
+    xyz = np.asarray(atoms[h_index][1], float) / ANGSTROM
+    sysm.charges = [(z, r) for z, r in sysm.charges if not np.allclose(r, xyz)]
+    sysm.species.append(custom_species("H_1", even_tempered_proton_shells(xyz)))
+    return sysm
+
+
+def serving_job(rng, proton_share: float = 0.2):
+    """One request of the served workload: (H2O)_n with n ~ U{1..24} placed
+    as the batched clusters, or C_nH_2n+2 with n ~ U{2..20} built as the
+    alkane workload (equal odds); 6-31G** or cc-pVDZ (equal odds); with
+    probability ``proton_share`` one hydrogen (chosen uniformly) is a quantum
+    proton.  Returns (label, MolecularSystem)."""
+    basis = "6-31G**" if rng.random() < 0.5 else "cc-pVDZ"
+    if rng.random() < 0.5:
+        n = int(rng.integers(1, 25))
+        atoms = water_cluster_geometry(n, rng)
+        label = "(H2O)%d/%s" % (n, basis)
+    else:
+        n = int(rng.integers(2, 21))
+        atoms = alkane_geometry(n)
+        label = "C%dH%d/%s" % (n, 2 * n + 2, basis)
+    if rng.random() < proton_share:
+        hs = [k for k, (sym, _) in enumerate(atoms) if sym == "H"]
+        k = hs[int(rng.integers(len(hs)))]
+        return label + "+H_1", with_quantum_proton(atoms, basis, k)
+    return label, build_system(atoms, basis)

@@ -61,3 +61,28 @@ def test_reference_energy_apmo_uhf_direct_on_gpu():
     print("   E = %.12f  diff %.2e  alpha-beta Hartree %.10f" % (E, E + 75.93125555356, ab))
     assert abs(E - (-75.93125555356)) <= 1e-8
     assert abs(ab - 23.2395577028) <= 1e-3
+
+
+@pytest.mark.parametrize("name,builder,eref", [
+    ("H2O.BOA.DIRECT", lambda: S.build_system(H2O, "6-311G"), -76.008843007734),
+    ("FHF-.NOCI-RHF", lambda: S.build_system(
+        [("F", (0, 0, -1.140079)), ("H", (0, 0, 0)), ("F", (0, 0, 1.140079))], "6-31G",
+        quantum=[("H_1", "NAKAI-3-S", (0, 0, 0), 1.0078250321 * 1822.88850065855 - 1.0)], extra_electrons=1),
+     -199.396255393556),
+    ("HCN", lambda: S.build_system([("H", (0, 0, 0)), ("C", (0, 0, 1.059)), ("N", (0, 0, 2.186))], "6-311++G**"),
+     -92.901743215468),
+    ("H2O.APMO.UHF.DIRECT", lambda: S.build_system(
+        H2O, "6-311G", quantum=[("H-A_1", "NAKAI-5-SPD", (0.0, 0.754175, 0.528381), 1836.15267247),
+                                ("H-B_1", "NAKAI-5-SPD", (0.0, -0.754174, 0.528382), 1836.15267247)],
+        open_shell=True), -75.93125555356),
+], ids=["H2O.BOA.DIRECT", "FHF-.NOCI-RHF", "HCN", "H2O.APMO.UHF.DIRECT"])
+def test_reference_energy_device_scf(name, builder, eref):
+    """The SCF with every matrix in HBM (rocSOLVER diagonalisation, device
+    DIIS) reaches the reference energies at their 1e-8 tolerance."""
+    from openlowdin_amd import scf_device as SD
+    _case("device-resident SCF: " + name)
+    scf = SD.DeviceSCF(builder())
+    E = scf.run()
+    print("   E = %.12f  ref %.12f  diff %.2e  (%d iterations)" % (E, eref, E - eref, scf.iterations))
+    assert scf.converged
+    assert abs(E - eref) <= 1e-8
