@@ -235,8 +235,17 @@ int lowdin_scf_start(void* scf);
  * current densities.  external_g != 0: the G buffers (lowdin_scf_buffers)
  * already hold J + f K of the current densities (a batched build). */
 int lowdin_scf_fock(void* scf, int external_g, double* energy, double* error);
+/* lowdin_scf_fock in two halves: _begin enqueues the step (nothing waits
+ * for the device), _end waits for its scalars and returns energy and error.
+ * Many SCF objects on one stream: begin all, then end all (one wait). */
+int lowdin_scf_fock_begin(void* scf, int external_g);
+int lowdin_scf_fock_end(void* scf, double* energy, double* error);
 /* DIIS extrapolation and new densities. */
 int lowdin_scf_update(void* scf);
+/* lowdin_scf_update for many SCF objects on one stream: their orthogonalised
+ * Fock matrices of equal dimension are diagonalised by one batched rocSOLVER
+ * call per dimension. */
+int lowdin_scf_update_many(void* const* scfs, int count);
 /* Iterate fock/update until |dE| < e_tol and error < d_tol.  Returns 0 when
  * converged, 1 when max_iter was reached, -1 on failure. */
 int lowdin_scf_run(void* scf, int max_iter, double e_tol, double d_tol, double* energy, int* iterations);

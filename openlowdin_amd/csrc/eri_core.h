@@ -180,8 +180,9 @@ __device__ __forceinline__ void contracted_quartet_body(BP bprim, int nbp, KP kp
   C::template hrr<double>(acc, ABx, ABy, ABz, CDx, CDy, CDz, out);
 }
 
-// FP32 Boys: fifth-order Taylor on the same grid (|dT| <= 0.05, error
-// ~3e-9 relative), exp(-T) the same way, downward recursion.
+// FP32 Boys: fifth-order Taylor on the full table's grid (|dT| <= 0.05,
+// error ~3e-9 relative), exp(-T) the same way, downward recursion; above
+// T = 40 the asymptotic form with upward recursion.
 template <int L>
 __device__ __forceinline__ void boys_f32(float T, const float* __restrict__ tab, float* F) {
   if (T < (float)kBoysTmax) {
@@ -268,12 +269,11 @@ __device__ __forceinline__ void contracted_quartet_f32_body(BP bprim, int nbp, K
 // here) keep FP64 in mixed mode, which halves their code and compile time.
 constexpr int kMaxNaccFp32 = 300;
 
-template <int LA, int LB, int LC, int LD, int RS = kBoysCols, int FO = 1 + LA + LB + LC + LD, typename BP,
-          typename KP, typename TP>
+template <int LA, int LB, int LC, int LD, typename BP, typename KP, typename TP>
 __device__ __forceinline__ void contracted_quartet(BP bprim, int nbp, KP kprim, int nkp, TP boys, double ABx,
                                                    double ABy, double ABz, double CDx, double CDy, double CDz,
                                                    double* out) {
-  contracted_quartet_body<LA, LB, LC, LD, RS, FO>(bprim, nbp, kprim, nkp, boys, ABx, ABy, ABz, CDx, CDy, CDz, out);
+  contracted_quartet_body<LA, LB, LC, LD>(bprim, nbp, kprim, nkp, boys, ABx, ABy, ABz, CDx, CDy, CDz, out);
 }
 
 template <int LA, int LB, int LC, int LD>
@@ -483,10 +483,6 @@ __global__ __launch_bounds__(256) void jk_strip_kernel(JKArgs a) {
   constexpr bool kF32 = Fp32Allowed<LA, LB, LC, LD>::value;
   extern __shared__ double strip[];  // rows of a (NA x n), then rows of b (NB x n)
   __shared__ double bpd[kMaxStripPrims * kPrimPairStride];
-#ifdef LOWDIN_LDS_BOYS
-  constexpr int LT = LA + LB + LC + LD;
-  __shared__ double btab[kBoysRows * 9];   // e^{-T_i}, F_LT .. F_LT+7 of this class
-#endif
   __shared__ unsigned int wg_q, wg_p, wg_q32;
   const int tid = threadIdx.x;
   const Task t = a.tasks[blockIdx.x];
@@ -505,12 +501,6 @@ __global__ __launch_bounds__(256) void jk_strip_kernel(JKArgs a) {
     const gcptr<double> src = prim + (size_t)P.poff * kPrimPairStride;
     for (int k = tid; k < P.nprim * kPrimPairStride; k += blockDim.x) bpd[k] = src[k];
   }
-#ifdef LOWDIN_LDS_BOYS
-  for (int k = tid; k < kBoysRows * 9; k += blockDim.x) {
-    const int r = k / 9, c = k - 9 * r;
-    btab[k] = a.boys[r * kBoysCols + (c == 0 ? 0 : LT + c)];
-  }
-#endif
   if (tid == 0) {
     wg_q = 0;
     wg_p = 0;
@@ -553,15 +543,9 @@ __global__ __launch_bounds__(256) void jk_strip_kernel(JKArgs a) {
       }
     }
     if (!lo) {
-#ifdef LOWDIN_LDS_BOYS
-      contracted_quartet<LA, LB, LC, LD, 9, 1>((const double*)bpd, P.nprim,
-                                               prim + (size_t)K.poff * kPrimPairStride, K.nprim, (const double*)btab,
-                                               P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
-#else
       contracted_quartet<LA, LB, LC, LD>((const double*)bpd, P.nprim,
                                          prim + (size_t)K.poff * kPrimPairStride, K.nprim, a.boys,
                                          P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
-#endif
     }
     const double deg = sbra * (K.sa != K.sb ? 2.0 : 1.0) * ((a.same_class && j == t.bra) ? 1.0 : 2.0);
     double nc[NC], nd[ND];

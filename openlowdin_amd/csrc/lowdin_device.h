@@ -32,7 +32,8 @@ constexpr int kBoysCols = 1 + kBoysMmax + kBoysTaylor;  // exp + F_0..F_{mmax+7}
 constexpr double kBoysH = 0.1;
 constexpr double kBoysInvH = 10.0;
 constexpr double kBoysTmax = 40.0;
-constexpr int kBoysRows = 401;           // T = 0 .. 40 inclusive
+constexpr int kBoysRows = 401;
+           // T = 0 .. 40 inclusive
 
 struct ShellInfo {
   int l;

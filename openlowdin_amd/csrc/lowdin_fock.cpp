@@ -81,8 +81,8 @@ struct Context {
   hipStream_t side[kSideStreams] = {};
   hipEvent_t fork = nullptr, join[kSideStreams] = {};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  DevBuf<double> boys;
-  DevBuf<float> boysf;
+  DevBuf<double> boys;    // full table (one-body integrals)
+  DevBuf<float> boysf;    // FP32 copy of the full table (mixed precision)
   std::mutex mu;  // one process stream: calls are serialised
 };
 
@@ -91,24 +91,29 @@ static Context& ctx() {
   return c;
 }
 
-// F_m(T_i) on the grid: power series for the highest order, then downward
-// recursion (all terms positive, no cancellation).
+// F_m(T) for m = 0..M at one point: power series for the highest order,
+// then downward recursion (all terms positive, no cancellation).
+static void boys_point(double T, int M, double* F) {
+  const double eT = std::exp(-T);
+  double term = 1.0 / (2 * M + 1), sum = term;
+  for (int k = 1; k < 20000; ++k) {
+    term *= 2.0 * T / (2 * M + 2 * k + 1);
+    sum += term;
+    if (term < 1e-18 * sum) break;
+  }
+  F[M] = eT * sum;
+  for (int m = M - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + eT) / (2 * m + 1);
+}
+
+// Full table (one-body integrals): per row T_i, e^{-T_i} then F_0..F_15.
 static std::vector<double> make_boys_table() {
   std::vector<double> t((size_t)kBoysRows * kBoysCols);
   const int M = kBoysCols - 2;
   double F[kBoysCols];
   for (int i = 0; i < kBoysRows; ++i) {
     const double T = i * kBoysH;
-    const double eT = std::exp(-T);
-    double term = 1.0 / (2 * M + 1), sum = term;
-    for (int k = 1; k < 4000; ++k) {
-      term *= 2.0 * T / (2 * M + 2 * k + 1);
-      sum += term;
-      if (term < 1e-18 * sum) break;
-    }
-    F[M] = eT * sum;
-    for (int m = M - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + eT) / (2 * m + 1);
-    t[(size_t)i * kBoysCols] = eT;
+    boys_point(T, M, F);
+    t[(size_t)i * kBoysCols] = std::exp(-T);
     for (int m = 0; m <= M; ++m) t[(size_t)i * kBoysCols + 1 + m] = F[m];
   }
   return t;
@@ -130,7 +135,7 @@ static void ensure_context() {
   HIPCHK(hipEventCreate(&c.ev1));
   std::vector<double> tab = make_boys_table();
   c.boys.upload(tab.data(), tab.size(), c.stream);
-  std::vector<float> tabf(tab.begin(), tab.end());
+  std::vector<float> tabf(tab.begin(), tab.end());   // FP32 copy of the full table (mixed precision)
   c.boysf.upload(tabf.data(), tabf.size(), c.stream);
   HIPCHK(hipStreamSynchronize(c.stream));
   c.ready = true;

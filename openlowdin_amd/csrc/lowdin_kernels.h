@@ -133,6 +133,7 @@ void launch_fock_assemble(double* F, const double* H, const double* G, double qq
 void launch_antisym(double* E, const double* T, int n, hipStream_t s);
 void launch_sym_add(double* Acc, const double* R, int n, double scale, hipStream_t s);
 void launch_lincomb(double* out, const double* hist, size_t nn, const LinComb& lc, hipStream_t s);
+void launch_absmax(const double* x, size_t n, double* out, hipStream_t s);
 void launch_scale_cols(double* X, const double* U, const double* w, int n, int m, int col0, hipStream_t s);
 void launch_symmetrize_batch(const SymDesc* d, int nmol, int nmax, hipStream_t s);
 void launch_fill_zero(double* p, size_t count, hipStream_t s);

@@ -298,6 +298,20 @@ __global__ void scale_cols_kernel(double* X, const double* U, const double* w, i
   X[(size_t)j * n + i] = U[(size_t)(col0 + j) * n + i] / sqrt(w[col0 + j]);
 }
 
+// out[0] = max |x| over n elements (one workgroup).
+__global__ __launch_bounds__(1024) void absmax_kernel(const double* x, size_t n, double* out) {
+  __shared__ double s[1024];
+  double m = 0.0;
+  for (size_t i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, fabs(x[i]));
+  s[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = s[0];
+}
+
 // Per class launch (blockIdx.x): sum the per-task counters of its task range
 // (ranges[2k], ranges[2k+1] = first task, task count) into out[3k .. 3k+2].
 __global__ __launch_bounds__(1024) void reduce_counts_multi_kernel(const unsigned int* counts, const int* ranges,
@@ -442,6 +456,10 @@ void launch_lincomb(double* out, const double* hist, size_t nn, const LinComb& l
 void launch_scale_cols(double* X, const double* U, const double* w, int n, int m, int col0, hipStream_t s) {
   if (n <= 0 || m <= 0) return;
   hipLaunchKernelGGL(scale_cols_kernel, dim3((n + 255) / 256, m), dim3(256), 0, s, X, U, w, n, m, col0);
+}
+
+void launch_absmax(const double* x, size_t n, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(absmax_kernel, dim3(1), dim3(1024), 0, s, x, n, out);
 }
 
 }  // namespace lowdin

@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -84,6 +85,11 @@ struct ScfSpecies {
   bool have_onebody = false;
 };
 
+// Per species, the scalars of one Fock step land in a device array and come
+// back with one asynchronous copy: Tr(D H), Tr(D G), Tr(D C), max |error|,
+// then the DIIS row <E_i, E_new> (kMaxDiis).
+constexpr int kScal = 5 + kMaxDiis;   // ..., [4] info of the last diagonalisation, [5..] DIIS row
+
 struct Scf {
   std::vector<ScfSpecies> sp;
   double e_pc = 0.0;
@@ -91,6 +97,10 @@ struct Scf {
   rocblas_handle blas = nullptr;
   int* d_info = nullptr;
   double* d_scratch = nullptr;   // kMaxDiis doubles (DIIS row)
+  double* d_sc = nullptr;        // kScal per species
+  double* h_sc = nullptr;        // pinned copy
+  hipEvent_t sc_ready = nullptr;
+  int pending_slot = -1;         // history slot of the Fock step in flight
   // DIIS ring
   int nhist = 0, head = 0;
   double B[kMaxDiis][kMaxDiis] = {};
@@ -104,6 +114,9 @@ struct Scf {
     if (own_stream && stream) (void)hipStreamDestroy(stream);
     if (d_info) (void)hipFree(d_info);
     if (d_scratch) (void)hipFree(d_scratch);
+    if (d_sc) (void)hipFree(d_sc);
+    if (h_sc) (void)hipHostFree(h_sc);
+    if (sc_ready) (void)hipEventDestroy(sc_ready);
   }
 };
 
@@ -127,18 +140,34 @@ void syevd(Scf& s, ScfSpecies& p, int n, double* A, double* W) {
   if (info != 0) throw std::runtime_error("rocsolver_dsyevd did not converge (info " + std::to_string(info) + ")");
 }
 
-// D = eta C_occ C_occ^T from the eigenvectors of X^T F X.
-void density_from_fock(Scf& s, ScfSpecies& p, const double* F) {
+// F' = X^T F X into Fp (m x m, leading dimension m).
+void orthogonal_fock(Scf& s, ScfSpecies& p, const double* F, double* Fp) {
   const int n = p.n, m = p.m;
   const rocblas_operation N = rocblas_operation_none, T = rocblas_operation_transpose;
   gemm(s, N, N, n, m, n, 1.0, F, n, p.X.p, n, 0.0, p.T.p, n);        // T = F X (n x m)
-  gemm(s, T, N, m, m, n, 1.0, p.X.p, n, p.T.p, n, 0.0, p.Fp.p, m);   // F' = X^T T (m x m)
-  syevd(s, p, m, p.Fp.p, p.W.p);
-  gemm(s, N, N, n, m, m, 1.0, p.X.p, n, p.Fp.p, m, 0.0, p.C.p, n);   // C = X C'
+  gemm(s, T, N, m, m, n, 1.0, p.X.p, n, p.T.p, n, 0.0, Fp, m);       // F' = X^T T
+}
+
+// C = X C', D = eta C_occ C_occ^T from the eigenvectors C' (m x m).
+void density_from_vectors(Scf& s, ScfSpecies& p, const double* Cp) {
+  const int n = p.n, m = p.m;
+  const rocblas_operation N = rocblas_operation_none, T = rocblas_operation_transpose;
+  gemm(s, N, N, n, m, m, 1.0, p.X.p, n, Cp, m, 0.0, p.C.p, n);
   if (p.nocc > 0)
     gemm(s, N, T, n, n, p.nocc, p.eta, p.C.p, n, p.C.p, n, 0.0, p.D.p, n);
   else
     lowdin::launch_fill_zero(p.D.p, (size_t)n * n, s.stream);
+}
+
+// D from F: orthogonal F', rocSOLVER dsyevd (its info lands in the species'
+// scalar slot and is checked with the next Fock step), C, D.  No host wait.
+void density_from_fock(Scf& s, size_t k, const double* F) {
+  ScfSpecies& p = s.sp[k];
+  orthogonal_fock(s, p, F, p.Fp.p);
+  BLASCHK(rocsolver_dsyevd(s.blas, rocblas_evect_original, rocblas_fill_upper, p.m, p.Fp.p, p.m, p.W.p, p.Ework.p,
+                           s.d_info));
+  SCFCHK(hipMemcpyAsync(s.d_sc + k * kScal + 4, s.d_info, sizeof(int), hipMemcpyDeviceToDevice, s.stream));
+  density_from_vectors(s, p, p.Fp.p);
 }
 
 void start(Scf& s) {
@@ -157,18 +186,20 @@ void start(Scf& s) {
     p.m = n - col0;
     if (p.m < p.nocc) throw std::runtime_error("fewer orthogonal functions than occupied orbitals");
     lowdin::launch_scale_cols(p.X.p, p.T.p, p.W.p, n, p.m, col0, s.stream);
-    // H-core guess
-    density_from_fock(s, p, p.H.p);
   }
+  // H-core guess
+  SCFCHK(hipMemsetAsync(s.d_sc, 0, s.sp.size() * kScal * sizeof(double), s.stream));
+  for (size_t k = 0; k < s.sp.size(); ++k) density_from_fock(s, k, s.sp[k].H.p);
   s.nhist = 0;
   s.head = 0;
   s.started = true;
 }
 
-// F, E and the DIIS error from the current densities; the error enters the
-// DIIS history (with F).  external_g: the species' G buffers were filled by
-// the caller (a batched build) for the current densities.
-void fock(Scf& s, bool external_g) {
+// Enqueue F, the energy terms and the DIIS error of the current densities;
+// the error enters the DIIS history (with F).  external_g: the species' G
+// buffers were filled by the caller (a batched build) for the current
+// densities.  Nothing waits for the device; fock_end collects the scalars.
+void fock_begin(Scf& s, bool external_g) {
   const size_t ns = s.sp.size();
   for (size_t k = 0; k < ns; ++k) {
     ScfSpecies& p = s.sp[k];
@@ -193,43 +224,63 @@ void fock(Scf& s, bool external_g) {
       lowdin::launch_sym_add(pb.Cs.p, pb.R.p, pb.n, qq, s.stream);
     }
   const int slot = s.nhist < kMaxDiis ? s.nhist : s.head;
-  double energy = s.e_pc, err = 0.0;
-  std::vector<double> row(kMaxDiis, 0.0);
+  const int nstored = std::min(s.nhist + 1, kMaxDiis);
+  BLASCHK(rocblas_set_pointer_mode(s.blas, rocblas_pointer_mode_device));
   for (size_t k = 0; k < ns; ++k) {
     ScfSpecies& p = s.sp[k];
     const int n = p.n, m = p.m;
     const size_t nn = (size_t)n * n, mm = (size_t)m * m;
+    double* sc = s.d_sc + k * kScal;
     const double qq = p.charge * p.charge;
     lowdin::launch_fock_assemble(p.F.p, p.H.p, p.G.p, qq, p.Cs.p, nn, s.stream);
-    energy += dot(s, nn, p.D.p, p.H.p) + 0.5 * qq * dot(s, nn, p.D.p, p.G.p) + 0.5 * dot(s, nn, p.D.p, p.Cs.p);
+    BLASCHK(rocblas_ddot(s.blas, (rocblas_int)nn, p.D.p, 1, p.H.p, 1, sc + 0));
+    BLASCHK(rocblas_ddot(s.blas, (rocblas_int)nn, p.D.p, 1, p.G.p, 1, sc + 1));
+    BLASCHK(rocblas_ddot(s.blas, (rocblas_int)nn, p.D.p, 1, p.Cs.p, 1, sc + 2));
     // error = X^T (F D S - S D F) X; (F D S)^T = S D F for symmetric F, D, S
     const rocblas_operation N = rocblas_operation_none, T = rocblas_operation_transpose;
+    const double one = 1.0, zero = 0.0;
+    BLASCHK(rocblas_set_pointer_mode(s.blas, rocblas_pointer_mode_host));
     gemm(s, N, N, n, n, n, 1.0, p.F.p, n, p.D.p, n, 0.0, p.T.p, n);
     gemm(s, N, N, n, n, n, 1.0, p.T.p, n, p.S.p, n, 0.0, p.R.p, n);
     lowdin::launch_antisym(p.T.p, p.R.p, n, s.stream);
     gemm(s, T, N, m, n, n, 1.0, p.X.p, n, p.T.p, n, 0.0, p.R.p, m);     // X^T A (m x n)
     double* e = p.Eh.p + (size_t)slot * mm;
     gemm(s, N, N, m, m, n, 1.0, p.R.p, m, p.X.p, n, 0.0, e, m);        // (X^T A) X
-    rocblas_int imax = 0;
-    BLASCHK(rocblas_idamax(s.blas, (rocblas_int)mm, e, 1, &imax));
-    double emax = 0.0;
-    if (imax > 0) {
-      SCFCHK(hipMemcpyAsync(&emax, e + (imax - 1), sizeof(double), hipMemcpyDeviceToHost, s.stream));
-      SCFCHK(hipStreamSynchronize(s.stream));
-    }
-    err = std::max(err, std::fabs(emax));
+    lowdin::launch_absmax(e, mm, sc + 3, s.stream);
     SCFCHK(hipMemcpyAsync(p.Fh.p + (size_t)slot * nn, p.F.p, nn * sizeof(double), hipMemcpyDeviceToDevice, s.stream));
-    // B row: <E_i, E_slot> for every stored i (one gemv over the history)
-    const int nstored = std::min(s.nhist + 1, kMaxDiis);
-    const double one = 1.0, zero = 0.0;
+    // DIIS row: <E_i, E_slot> for every stored i (one gemv over the history)
     BLASCHK(rocblas_dgemv(s.blas, rocblas_operation_transpose, (rocblas_int)mm, nstored, &one, p.Eh.p,
-                          (rocblas_int)mm, e, 1, &zero, s.d_scratch, 1));
-    std::vector<double> part(nstored);
-    SCFCHK(hipMemcpyAsync(part.data(), s.d_scratch, nstored * sizeof(double), hipMemcpyDeviceToHost, s.stream));
-    SCFCHK(hipStreamSynchronize(s.stream));
-    for (int i = 0; i < nstored; ++i) row[i] += part[i];
+                          (rocblas_int)mm, e, 1, &zero, sc + 5, 1));
+    BLASCHK(rocblas_set_pointer_mode(s.blas, rocblas_pointer_mode_device));
   }
+  BLASCHK(rocblas_set_pointer_mode(s.blas, rocblas_pointer_mode_host));
+  SCFCHK(hipMemcpyAsync(s.h_sc, s.d_sc, ns * kScal * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+  SCFCHK(hipEventRecord(s.sc_ready, s.stream));
+  s.pending_slot = slot;
+}
+
+// Wait for the scalars of fock_begin; energy, error, DIIS bookkeeping.
+void fock_end(Scf& s) {
+  if (s.pending_slot < 0) throw std::runtime_error("no Fock step in flight");
+  SCFCHK(hipEventSynchronize(s.sc_ready));
+  const int slot = s.pending_slot;
+  s.pending_slot = -1;
+  const size_t ns = s.sp.size();
   const int nstored = std::min(s.nhist + 1, kMaxDiis);
+  double energy = s.e_pc, err = 0.0;
+  std::vector<double> row(kMaxDiis, 0.0);
+  for (size_t k = 0; k < ns; ++k) {
+    const ScfSpecies& p = s.sp[k];
+    const double* sc = s.h_sc + k * kScal;
+    const double qq = p.charge * p.charge;
+    energy += sc[0] + 0.5 * qq * sc[1] + 0.5 * sc[2];
+    err = std::max(err, sc[3]);
+    int info = 0;
+    std::memcpy(&info, sc + 4, sizeof(int));
+    if (info != 0)
+      throw std::runtime_error("rocsolver_dsyevd did not converge (info " + std::to_string(info) + ")");
+    for (int i = 0; i < nstored; ++i) row[i] += sc[5 + i];
+  }
   for (int i = 0; i < nstored; ++i) {
     s.B[slot][i] = row[i];
     s.B[i][slot] = row[i];
@@ -240,6 +291,11 @@ void fock(Scf& s, bool external_g) {
     s.head = (s.head + 1) % kMaxDiis;
   s.energy = energy;
   s.error = err;
+}
+
+void fock(Scf& s, bool external_g) {
+  fock_begin(s, external_g);
+  fock_end(s);
 }
 
 // Solve [B -1; -1 0] [c; l] = [0; -1] (Gaussian elimination, partial
@@ -277,8 +333,8 @@ bool diis_solve(int m, const double B[kMaxDiis][kMaxDiis], const int* idx, doubl
   return true;
 }
 
-// DIIS extrapolation over the stored Fock matrices, then new densities.
-void update(Scf& s) {
+// DIIS extrapolation over the stored Fock matrices into F.
+void extrapolate(Scf& s) {
   const int m = s.nhist;
   int idx[kMaxDiis];
   // chronological order: oldest first
@@ -294,10 +350,72 @@ void update(Scf& s) {
     lc.c[i] = c[i];
     lc.idx[i] = idx[i];
   }
-  for (ScfSpecies& p : s.sp) {
-    const size_t nn = (size_t)p.n * p.n;
-    lowdin::launch_lincomb(p.F.p, p.Fh.p, nn, lc, s.stream);
-    density_from_fock(s, p, p.F.p);
+  for (size_t k = 0; k < s.sp.size(); ++k) {
+    ScfSpecies& p = s.sp[k];
+    lowdin::launch_lincomb(p.F.p, p.Fh.p, (size_t)p.n * p.n, lc, s.stream);
+  }
+}
+
+void update(Scf& s) {
+  extrapolate(s);
+  for (size_t k = 0; k < s.sp.size(); ++k) density_from_fock(s, k, s.sp[k].F.p);
+}
+
+// Grouped diagonalisation for many SCF objects on one stream: the
+// orthogonalised Fock matrices of equal dimension m are packed into one
+// strided array and diagonalised by one rocsolver_dsyevd_strided_batched.
+struct GroupWork {
+  Buf A, W, E;
+  int* info = nullptr;
+  int cap = 0;
+  ~GroupWork() {
+    if (info) (void)hipFree(info);
+  }
+};
+
+void update_many(const std::vector<Scf*>& scfs) {
+  static std::map<int, GroupWork> work;   // per dimension m, kept for the process
+  hipStream_t st = scfs[0]->stream;
+  for (Scf* s : scfs)
+    if (s->stream != st) {
+      for (Scf* t : scfs) update(*t);
+      return;
+    }
+  std::map<int, std::vector<std::pair<Scf*, size_t>>> groups;
+  for (Scf* s : scfs) {
+    extrapolate(*s);
+    for (size_t k = 0; k < s->sp.size(); ++k) groups[s->sp[k].m].push_back({s, k});
+  }
+  Scf& s0 = *scfs[0];
+  for (auto& g : groups) {
+    const int m = g.first;
+    const int cnt = (int)g.second.size();
+    const size_t mm = (size_t)m * m;
+    GroupWork& w = work[m];
+    if (w.cap < cnt) {
+      SCFCHK(hipStreamSynchronize(st));
+      w.A.alloc((size_t)cnt * mm);
+      w.W.alloc((size_t)cnt * m);
+      w.E.alloc((size_t)cnt * m);
+      if (w.info) (void)hipFree(w.info);
+      w.info = nullptr;
+      SCFCHK(hipMalloc((void**)&w.info, cnt * sizeof(int)));
+      w.cap = cnt;
+    }
+    for (int i = 0; i < cnt; ++i) {
+      Scf& s = *g.second[i].first;
+      ScfSpecies& p = s.sp[g.second[i].second];
+      orthogonal_fock(s, p, p.F.p, w.A.p + (size_t)i * mm);
+    }
+    BLASCHK(rocsolver_dsyevd_strided_batched(s0.blas, rocblas_evect_original, rocblas_fill_upper, m, w.A.p, m,
+                                             (rocblas_stride)mm, w.W.p, m, w.E.p, m, w.info, cnt));
+    for (int i = 0; i < cnt; ++i) {
+      Scf& s = *g.second[i].first;
+      const size_t k = g.second[i].second;
+      ScfSpecies& p = s.sp[k];
+      SCFCHK(hipMemcpyAsync(s.d_sc + k * kScal + 4, w.info + i, sizeof(int), hipMemcpyDeviceToDevice, st));
+      density_from_vectors(s, p, w.A.p + (size_t)i * mm);
+    }
   }
 }
 
@@ -323,6 +441,9 @@ void* lowdin_scf_new(int nspecies, void* const* lints, const lowdin_scf_species*
     BLASCHK(rocblas_set_pointer_mode(s->blas, rocblas_pointer_mode_host));
     SCFCHK(hipMalloc((void**)&s->d_info, sizeof(int)));
     SCFCHK(hipMalloc((void**)&s->d_scratch, kMaxDiis * sizeof(double)));
+    SCFCHK(hipMalloc((void**)&s->d_sc, (size_t)nspecies * kScal * sizeof(double)));
+    SCFCHK(hipHostMalloc((void**)&s->h_sc, (size_t)nspecies * kScal * sizeof(double), hipHostMallocDefault));
+    SCFCHK(hipEventCreateWithFlags(&s->sc_ready, hipEventDisableTiming));
     s->sp.resize(nspecies);
     for (int k = 0; k < nspecies; ++k) {
       ScfSpecies& p = s->sp[k];
@@ -393,6 +514,47 @@ int lowdin_scf_fock(void* scf, int external_g, double* energy, double* error) {
     return 0;
   } catch (const std::exception& e) {
     lowdin::set_last_error(std::string("scf_fock: ") + e.what());
+    return -1;
+  }
+}
+
+int lowdin_scf_fock_begin(void* scf, int external_g) {
+  try {
+    Scf& s = *as_scf(scf);
+    if (!s.started) throw std::runtime_error("lowdin_scf_start first");
+    fock_begin(s, external_g != 0);
+    lowdin::clear_last_error();
+    return 0;
+  } catch (const std::exception& e) {
+    lowdin::set_last_error(std::string("scf_fock_begin: ") + e.what());
+    return -1;
+  }
+}
+
+int lowdin_scf_fock_end(void* scf, double* energy, double* error) {
+  try {
+    Scf& s = *as_scf(scf);
+    fock_end(s);
+    if (energy) *energy = s.energy;
+    if (error) *error = s.error;
+    lowdin::clear_last_error();
+    return 0;
+  } catch (const std::exception& e) {
+    lowdin::set_last_error(std::string("scf_fock_end: ") + e.what());
+    return -1;
+  }
+}
+
+int lowdin_scf_update_many(void* const* scfs, int count) {
+  try {
+    if (count <= 0) return 0;
+    std::vector<Scf*> v;
+    for (int i = 0; i < count; ++i) v.push_back(as_scf(scfs[i]));
+    update_many(v);
+    lowdin::clear_last_error();
+    return 0;
+  } catch (const std::exception& e) {
+    lowdin::set_last_error(std::string("scf_update_many: ") + e.what());
     return -1;
   }
 }

@@ -125,6 +125,18 @@ class Stream:
     def synchronize(self):
         check(hip().hipStreamSynchronize(self.handle), "hipStreamSynchronize")
 
+    def destroy(self):
+        if getattr(self, "handle", None):
+            hip().hipStreamSynchronize(self.handle)
+            hip().hipStreamDestroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
 
 class Event:
     def __init__(self):

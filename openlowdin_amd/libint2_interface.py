@@ -120,6 +120,12 @@ def lib():
         L.lowdin_scf_fock.restype = C.c_int
         L.lowdin_scf_update.argtypes = [vp]
         L.lowdin_scf_update.restype = C.c_int
+        L.lowdin_scf_fock_begin.argtypes = [vp, C.c_int]
+        L.lowdin_scf_fock_begin.restype = C.c_int
+        L.lowdin_scf_fock_end.argtypes = [vp, dp, dp]
+        L.lowdin_scf_fock_end.restype = C.c_int
+        L.lowdin_scf_update_many.argtypes = [C.POINTER(vp), C.c_int]
+        L.lowdin_scf_update_many.restype = C.c_int
         L.lowdin_scf_run.argtypes = [vp, C.c_int, C.c_double, C.c_double, dp, ip]
         L.lowdin_scf_run.restype = C.c_int
         L.lowdin_scf_get.argtypes = [vp, C.c_int, C.c_int, dp]

@@ -70,6 +70,16 @@ class DeviceSCF:
             LI._check("scf_fock")
         return e.value, err.value
 
+    def fock_begin(self, external_g: bool = False):
+        if LI.lib().lowdin_scf_fock_begin(self.this, 1 if external_g else 0) != 0:
+            LI._check("scf_fock_begin")
+
+    def fock_end(self):
+        e, err = C.c_double(0.0), C.c_double(0.0)
+        if LI.lib().lowdin_scf_fock_end(self.this, C.byref(e), C.byref(err)) != 0:
+            LI._check("scf_fock_end")
+        return e.value, err.value
+
     def update(self):
         if LI.lib().lowdin_scf_update(self.this) != 0:
             LI._check("scf_update")
@@ -106,3 +116,13 @@ class DeviceSCF:
     @property
     def stream(self) -> int:
         return LI.lib().lowdin_scf_stream(self.this) or 0
+
+
+def update_many(scfs):
+    """DIIS + diagonalisation for many SCF objects on one stream, the
+    diagonalisations batched per matrix dimension."""
+    if not scfs:
+        return
+    arr = (C.c_void_p * len(scfs))(*[s.this for s in scfs])
+    if LI.lib().lowdin_scf_update_many(arr, len(scfs)) != 0:
+        LI._check("scf_update_many")

@@ -79,6 +79,8 @@ class ScfEngine:
         self._batch = None
         self._batch_key = None
         self.ticks = 0
+        self.build_ms = []          # device time of each tick's batched build
+        self._ev = (H.Event(), H.Event())
         self.t0 = time.perf_counter()
 
     def now(self) -> float:
@@ -134,10 +136,16 @@ class ScfEngine:
         self._admit()
         if not self.active:
             return
+        self._ev[0].record(self.stream)
         self._build()
-        keep = []
+        self._ev[1].record(self.stream)
         for job in self.active:
-            e, err = job.scf.fock(external_g=True)
+            job.scf.fock_begin(external_g=True)
+        keep = []
+        for k, job in enumerate(self.active):
+            e, err = job.scf.fock_end()
+            if k == 0:
+                self.build_ms.append(self._ev[0].elapsed_ms(self._ev[1]))
             job.iterations += 1
             conv = job.e_old is not None and abs(e - job.e_old) < self.e_tol and err < self.d_tol
             if conv or job.iterations >= self.max_iter:
@@ -150,8 +158,8 @@ class ScfEngine:
                 self.done.append(job)
             else:
                 job.e_old = e
-                job.scf.update()
                 keep.append(job)
+        SD.update_many([j.scf for j in keep])
         self.active = keep
         self.ticks += 1
 
@@ -172,4 +180,5 @@ class ScfEngine:
                 "p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
                 "p99_s": float(np.percentile(lat, 99)) if len(lat) else None,
                 "mean_iterations": float(np.mean([j.iterations for j in self.done])) if self.done else None,
-                "ticks": self.ticks}
+                "ticks": self.ticks,
+                "fock_build_ms_per_tick_mean": float(np.mean(self.build_ms)) if self.build_ms else None}
