@@ -78,6 +78,39 @@ void LibintInterface_compute_coupling_direct(void* lint, void* olint, double* de
  * n x n.  Computed on the GPU. */
 void LibintInterface_compute_1body_ints(void* lint, int integral_kind, double* result);
 
+/* Libint2Iface.cpp:3736-3750 (LibintInterface_compute_2body_directAll,
+ * compute_2body_directAll at :2552-2742).  results (n^4, element (i j|k l)
+ * at i n^3 + j n^2 + k n + l): every integral of the Fock build's quartet set
+ * (same pair list and Dnorm * Q * Q < DBL_EPSILON screen, dens giving the
+ * density block norms) with |raw| > 1e-10, normalised with norma, written to
+ * all eight index images.  Elements not written keep the caller's values. */
+void LibintInterface_compute_2body_directAll(void* lint, double* dens, double* results);
+
+/* Libint2Iface.cpp:3528-3549 (LibintInterface_compute_2body_directIT,
+ * compute_2body_directIT at :620-921).  First-quarter transform on orbital
+ * p: result[j n^2 + k n + l] = sum_i C(p, i) (i j|k l), C(p, i) = coeff[p +
+ * i n] (coeff n x n column-major, p passed by address as the Fortran binding
+ * does), over the same quartets, drop and screen as directAll.  Every
+ * element of result is written. */
+void LibintInterface_compute_2body_directIT(void* lint, double* dens, double* coeff, double* result, int* p);
+
+/* Libint2Iface.cpp:3752-3773 (LibintInterface_compute_coupling_directAll,
+ * compute_coupling_directAll at :2744-2908).  Every shell pair of both
+ * species, no screen, |raw| >= 1e-10: results[x1 n n_o^2 + x2 n_o^2 + y1 n_o
+ * + y2] = (x1 x2|y1 y2) normalised, x of the lower species id, y of the
+ * other, n = lint's basis size, n_o = olint's, four images (x1 <-> x2,
+ * y1 <-> y2).  Called from the higher species id the reference's indices
+ * only fit when n == n_o; otherwise the call fails.  dens is not read. */
+void LibintInterface_compute_coupling_directAll(void* lint, void* olint, double* dens, double* results);
+
+/* Libint2Iface.cpp:3591-3623 (LibintInterface_compute_coupling_directIT,
+ * compute_coupling_directIT at :1296-1622).  results[v n_o^2 + l n_o + s]
+ * (n x n_o x n_o) = sum_u C(p, u) (u v|l s), u, v of lint's species, l, s of
+ * olint's, C(p, u) = coeff[p + u n]; same quartets and drop as
+ * coupling_directAll.  Every element is written; dens is not read. */
+void LibintInterface_compute_coupling_directIT(void* lint, void* olint, double* dens, double* coeff, double* results,
+                                               int* p);
+
 /* ------------------------------------------------ extensions (no reference
  * counterpart; used by this package's host code, tests and benchmarks) */
 
@@ -124,6 +157,17 @@ int lowdin_fock_coupling_pair(void* a, void* b, const double* da, const double* 
  * call.  Returns 0 on success. */
 int lowdin_fock_coupling_pair_device(void* a, void* b, const double* d_da, const double* d_db, double* d_ra,
                                      double* d_rb, void* stream, int synchronize);
+
+/* Device-resident forms of the integral store / first-quarter transform:
+ * which 0 = 2body_directAll (d_dens n x n, d_out n^4, written where kept),
+ * 1 = 2body_directIT (d_dens, d_crow = C(p, :) as n contiguous values,
+ * d_out n^3), 2 = coupling_directAll (olint, d_out n^2 n_o^2, written where
+ * kept), 3 = coupling_directIT (olint, d_crow, d_out n n_o^2).  Layouts and
+ * element sets as the host entry points above; runs on the library stream
+ * and synchronises.  Counters in lowdin_fock_get_stats count the candidate
+ * quartets before the density screen.  Returns 0 on success. */
+int lowdin_fock_store_device(void* lint, void* olint, int which, const double* d_dens, const double* d_crow,
+                             double* d_out);
 
 /* Device-resident variant of LibintInterface_compute_2body_direct: d_dens
  * and d_result are device pointers (n x n FP64); the build is enqueued on

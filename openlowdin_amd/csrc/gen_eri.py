@@ -2,9 +2,10 @@
 
 For every canonical quartet class (la lb|lc ld) with l <= LMAX, la >= lb,
 lc >= ld and pair class (la, lb) >= (lc, ld), this emits two device
-functions into ``gen/eri_<class>.inc``, a kernel translation unit
-``gen/k_<class>.hip`` per class (so the classes compile in parallel), and the
-launch dispatch table ``gen/dispatch.h``:
+functions into ``gen/eri_<class>.inc``, two kernel translation units per
+class (``gen/k_<class>.hip``: Fock build, coupling and Schwarz kernels;
+``gen/s_<class>.hip``: integral store and first-quarter transform), so that
+they compile in parallel, and the launch dispatch table ``gen/dispatch.h``:
 
 ``vrr_<la><lb><lc><ld>(F, ...)``
     Obara-Saika vertical recursion (Obara & Saika, J. Chem. Phys. 84, 3963
@@ -225,6 +226,20 @@ void launch_cp_%(k)s(const CouplingArgs& args, hipStream_t s) { launch_cp_t<%(a)
 """
 
 
+STORE_TU = """// Generated by gen_eri.py -- do not edit.
+// Kernels of quartet class (%(a)d%(b)d|%(c)d%(d)d): integral store and first-quarter transform.
+#include "../eri_store.h"
+
+namespace lowdin {
+#include "eri_%(k)s.inc"
+LOWDIN_ERI_CLASS(%(a)d, %(b)d, %(c)d, %(d)d)
+
+void launch_st_%(k)s(const StoreArgs& args, hipStream_t s) { launch_store_t<%(a)d, %(b)d, %(c)d, %(d)d>(args, s); }
+void launch_cst_%(k)s(const CouplingStoreArgs& args, hipStream_t s) { launch_cstore_t<%(a)d, %(b)d, %(c)d, %(d)d>(args, s); }
+}  // namespace lowdin
+"""
+
+
 def main(outdir):
     pcs, classes = canonical_classes()
     gdir = os.path.join(outdir, "gen")
@@ -251,6 +266,8 @@ def main(outdir):
         with open(os.path.join(gdir, "k_%s.hip" % k), "w") as fh:
             fh.write(KERNEL_TU % {"a": la, "b": lb, "c": lc, "d": ld, "k": k,
                                   "schw": ", Schwarz" if diag else "", "schwfn": schwfn})
+        with open(os.path.join(gdir, "s_%s.hip" % k), "w") as fh:
+            fh.write(STORE_TU % {"a": la, "b": lb, "c": lc, "d": ld, "k": k})
         table[k] = {"L": L, "nacc": nacc, "vrr_flops": v.flops, "hrr_flops": h.flops}
     with open(os.path.join(outdir, "ops_table.json"), "w") as fh:
         json.dump({"note": "floating-point operations of the generated Obara-Saika VRR (per primitive "
@@ -264,6 +281,8 @@ def main(outdir):
         lines.append("void launch_jk_%s(const JKArgs&, hipStream_t);" % k)
         lines.append("void launch_jks_%s(const JKArgs&, hipStream_t);" % k)
         lines.append("void launch_cp_%s(const CouplingArgs&, hipStream_t);" % k)
+        lines.append("void launch_st_%s(const StoreArgs&, hipStream_t);" % k)
+        lines.append("void launch_cst_%s(const CouplingStoreArgs&, hipStream_t);" % k)
     for (la, lb) in pcs:
         lines.append("void launch_schwarz_%d%d(PairInfo*, int, const double*, const double*, const double*, hipStream_t);"
                      % (la, lb))
@@ -281,6 +300,16 @@ def main(outdir):
     lines.append("  switch (la * 1000 + lb * 100 + lc * 10 + ld) {")
     for (la, lb, lc, ld) in classes:
         lines.append("    case %d: launch_cp_%d%d%d%d(a, s); return true;" % (la * 1000 + lb * 100 + lc * 10 + ld, la, lb, lc, ld))
+    lines.append("    default: return false;\n  }\n}")
+    lines.append("inline bool dispatch_store(int la, int lb, int lc, int ld, const StoreArgs& a, hipStream_t s) {")
+    lines.append("  switch (la * 1000 + lb * 100 + lc * 10 + ld) {")
+    for (la, lb, lc, ld) in classes:
+        lines.append("    case %d: launch_st_%d%d%d%d(a, s); return true;" % (la * 1000 + lb * 100 + lc * 10 + ld, la, lb, lc, ld))
+    lines.append("    default: return false;\n  }\n}")
+    lines.append("inline bool dispatch_cstore(int la, int lb, int lc, int ld, const CouplingStoreArgs& a, hipStream_t s) {")
+    lines.append("  switch (la * 1000 + lb * 100 + lc * 10 + ld) {")
+    for (la, lb, lc, ld) in classes:
+        lines.append("    case %d: launch_cst_%d%d%d%d(a, s); return true;" % (la * 1000 + lb * 100 + lc * 10 + ld, la, lb, lc, ld))
     lines.append("    default: return false;\n  }\n}")
     lines.append("inline bool dispatch_schwarz(int la, int lb, PairInfo* p, int np, const double* prim, const double* boys, "
                  "const double* norma, hipStream_t s) {")

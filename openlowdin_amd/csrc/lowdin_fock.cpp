@@ -208,6 +208,9 @@ struct Species {
   std::map<uint64_t, CouplingKey> coupling_tasks;  // keyed by the other species' uid
 
   DevBuf<double> d_D, d_G, d_Dother, d_R;
+  TaskSet store_ts;                  // wavefront tasks of the integral store / transform
+  uint64_t store_gen = ~0ull;        // sig_gen the store tasks were built from
+  DevBuf<double> d_Dn1, d_cvec, d_store;
   lowdin_fock_stats stats{};
   int mixed = 0;            // mixed precision: FP32 quartets below tau
   double tau = 0.0;
@@ -944,6 +947,208 @@ static void compute_coupling(Species& a, Species* b, const double* db, double* r
   a.stats = stt;
 }
 
+// ------------------------------------------ integral store / first-quarter transform
+// Wavefront tasks over the significant pairs of one species: the quartet
+// set of the Fock build (bra class >= ket class, ket <= bra within a class).
+static TaskSet& store_taskset(Species& s, hipStream_t st) {
+  TaskSet& ts = s.store_ts;
+  if (ts.valid && s.store_gen == s.sig_gen) return ts;
+  ts.tasks.clear();
+  ts.launches.clear();
+  const PairSet& ps = s.sig;
+  for (int cx = 0; cx < kNumPairClasses; ++cx)
+    for (int cy = 0; cy <= cx; ++cy) {
+      const int nx = ps.off[cx + 1] - ps.off[cx], ny = ps.off[cy + 1] - ps.off[cy];
+      if (nx == 0 || ny == 0) continue;
+      const PairInfo* kets = ps.pairs.data() + ps.off[cy];
+      const size_t from = ts.tasks.size();
+      for (int i = 0; i < nx; ++i) emit_tasks(ts.tasks, i, 0, cx == cy ? i + 1 : ny, kets, 0);
+      sort_by_cost(ts.tasks, from, ps.pairs.data() + ps.off[cx], kets);
+      ts.launches.push_back({kPairClassL[cx][0], kPairClassL[cx][1], kPairClassL[cy][0], kPairClassL[cy][1], cx, cy,
+                             1, from, ts.tasks.size() - from, 0});
+    }
+  ts.d_tasks.upload(ts.tasks.data(), ts.tasks.size(), st);
+  ts.valid = true;
+  s.store_gen = s.sig_gen;
+  return ts;
+}
+
+// Candidate quartets / ERIs of a task range (before the density screen).
+static void count_tasks(const TaskSet& ts, const TaskSet::Launch& L, const PairInfo* bras, const PairInfo* kets,
+                        lowdin_fock_stats* stt) {
+  if (!stt) return;
+  const uint64_t per = (uint64_t)ncart_h(L.la) * ncart_h(L.lb) * ncart_h(L.lc) * ncart_h(L.ld);
+  for (size_t k = L.off; k < L.off + L.count; ++k) {
+    const Task& t = ts.tasks[k];
+    const int nq = t.j1 - t.j0;
+    stt->quartets += nq;
+    stt->eris += nq * per;
+    stt->prim_quartets += (uint64_t)nq * bras[t.bra].nprim * kets[t.j0].nprim;
+  }
+}
+
+// Intra-species integral store (mode 0: dOut = A, n^4, only kept elements
+// written) or first-quarter transform (mode 1: dOut = GG, n^3, zeroed,
+// accumulated and mirrored here; dC = row p of the coefficients), enqueued
+// on st from the device density dD (its block norms drive the screen).
+static void enqueue_store(Species& s, const double* dD, const double* dC, double* dOut, int mode, hipStream_t st,
+                          lowdin_fock_stats* stt) {
+  Context& c = ctx();
+  ensure_sig(s, st);
+  TaskSet& ts = store_taskset(s, st);
+  const int n = s.nbasis, ns = (int)s.shells.size();
+  const bool screen = s.is_electron;
+  if (screen) launch_dnorm(dD, n, s.d_shells.p, ns, s.d_Dn1.ensure((size_t)ns * ns), st);
+  if (mode == 1) launch_fill_zero(dOut, (size_t)n * n * n, st);
+  hipStream_t hs = c.side[0];
+  HIPCHK(hipEventRecord(c.fork, st));
+  HIPCHK(hipStreamWaitEvent(hs, c.fork, 0));
+  for (const TaskSet::Launch& L : ts.launches) {
+    StoreArgs g{};
+    g.tasks = ts.d_tasks.p + L.off;
+    g.ntasks = (int)L.count;
+    g.bra = s.sig.d_pairs.p + s.sig.off[L.cbra];
+    g.ket = s.sig.d_pairs.p + s.sig.off[L.cket];
+    g.prim = s.sig.d_prims.p;
+    g.norma = s.d_norma.p;
+    g.Dn = s.d_Dn1.p;
+    g.c = dC;
+    g.out = dOut;
+    g.boys = c.boys.p;
+    g.n = n;
+    g.nshell = ns;
+    g.same_class = L.cbra == L.cket ? 1 : 0;
+    g.screen = screen ? 1 : 0;
+    g.mode = mode;
+    g.eps = 2.220446049250313080847e-16;  // DBL_EPSILON, Libint2Iface.h:273-274, 316-317
+    g.drop = 1.0e-10;                      // Libint2Iface.cpp:794, 2701
+    if (!dispatch_store(L.la, L.lb, L.lc, L.ld, g, hs)) throw std::runtime_error("no store kernel for class");
+    count_tasks(ts, L, s.sig.pairs.data() + s.sig.off[L.cbra], s.sig.pairs.data() + s.sig.off[L.cket], stt);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(c.join[0], hs));
+  HIPCHK(hipStreamWaitEvent(st, c.join[0], 0));
+  if (mode == 1) launch_mirror_upper(dOut, n, n, st);
+  HIPCHK(hipGetLastError());
+}
+
+// Inter-species store / transform between the calling species a and the
+// other species b (LibintInterface_compute_coupling_directAll/IT with
+// lint = a, olint = b; permuted = a's species id > b's).
+//   mode 0: A[x1][x2][y1][y2], X the lower species id, strides
+//           (n_a n_b^2, n_b^2, n_b, 1) as the reference indexes it
+//           (Libint2Iface.cpp:2748-2749, 2877-2880); with permuted and
+//           n_a != n_b those indices leave the n_a^2 n_b^2 array, which is
+//           refused here.
+//   mode 1: GG[v][l][s] (n_a x n_b x n_b) = sum_u C(p,u) (uv|ls), u, v of a
+//           (Libint2Iface.cpp:1318-1330, 1495-1545).
+static void enqueue_coupling_store(Species& a, Species& b, const double* dC, double* dOut, int mode, hipStream_t st,
+                                   lowdin_fock_stats* stt) {
+  Context& c = ctx();
+  const bool permuted = a.species_id > b.species_id;
+  if (mode == 0 && permuted && a.nbasis != b.nbasis)
+    throw std::runtime_error("coupling_directAll: called from the higher species id with a different basis size "
+                             "(the reference layout indexes outside the n_a^2 n_b^2 result)");
+  ensure_all(a, st);
+  ensure_all(b, st);
+  TaskSet* ts = coupling_taskset(a, b, st);
+  const size_t na = a.nbasis, nb = b.nbasis;
+  if (mode == 1) launch_fill_zero(dOut, na * nb * nb, st);
+  hipStream_t hs = c.side[0];
+  HIPCHK(hipEventRecord(c.fork, st));
+  HIPCHK(hipStreamWaitEvent(hs, c.fork, 0));
+  for (const TaskSet::Launch& L : ts->launches) {
+    const bool a_bra = L.bra_is_a != 0;
+    Species& sb = a_bra ? a : b;
+    Species& sk = a_bra ? b : a;
+    CouplingStoreArgs g{};
+    g.tasks = ts->d_tasks.p + L.off;
+    g.ntasks = (int)L.count;
+    g.bra = sb.all.d_pairs.p + sb.all.off[L.cbra];
+    g.ket = sk.all.d_pairs.p + sk.all.off[L.cket];
+    g.bprim = sb.all.d_prims.p;
+    g.kprim = sk.all.d_prims.p;
+    g.nbra_norma = sb.d_norma.p;
+    g.nket_norma = sk.d_norma.p;
+    g.c = dC;
+    g.out = dOut;
+    g.boys = c.boys.p;
+    g.bra_is_x = (a_bra != permuted) ? 1 : 0;   // X = b when permuted
+    g.bra_is_t = a_bra ? 1 : 0;
+    g.mode = mode;
+    g.n_o = (int)nb;
+    g.sx1 = na * nb * nb;
+    g.sx2 = nb * nb;
+    g.sy1 = nb;
+    g.drop = 1.0e-10;  // Libint2Iface.cpp:1456, 2865
+    if (!dispatch_cstore(L.la, L.lb, L.lc, L.ld, g, hs)) throw std::runtime_error("no coupling store kernel for class");
+    count_tasks(*ts, L, sb.all.pairs.data() + sb.all.off[L.cbra], sk.all.pairs.data() + sk.all.off[L.cket], stt);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(c.join[0], hs));
+  HIPCHK(hipStreamWaitEvent(st, c.join[0], 0));
+  if (mode == 1) launch_mirror_upper(dOut, (int)na, (int)nb, st);
+  HIPCHK(hipGetLastError());
+}
+
+// Row p of an n x n column-major coefficient matrix, C(p, u) = coeff[p + u n]
+// (the Eigen copy of Libint2Iface.cpp:3535-3537, read as C(p, bf)).
+static std::vector<double> coeff_row(const double* coeff, int n, int p) {
+  if (p < 0 || p >= n) throw std::runtime_error("orbital index p out of range");
+  std::vector<double> r(n);
+  for (int u = 0; u < n; ++u) r[u] = coeff[p + (size_t)u * n];
+  return r;
+}
+
+// Host-pointer store / transform: uploads, runs, downloads, records stats.
+//   which 0: intra All, 1: intra IT, 2: coupling All, 3: coupling IT.
+static void run_store(Species& a, Species* b, int which, const double* dens, const double* coeff, int p,
+                      double* result) {
+  Context& c = ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  const double t0 = now_ms();
+  ensure_context();
+  hipStream_t st = c.stream;
+  const size_t na = a.nbasis, nb = b ? b->nbasis : 0;
+  const int mode = which & 1;
+  size_t count = 0;
+  switch (which) {
+    case 0: count = na * na * na * na; break;
+    case 1: count = na * na * na; break;
+    case 2: count = na * na * nb * nb; break;
+    default: count = na * nb * nb; break;
+  }
+  double* dOut = a.d_store.ensure(count);
+  // the store writes only the elements it keeps: the rest keep the caller's values
+  if (mode == 0) a.d_store.upload(result, count, st);
+  const double* dD = nullptr;
+  if (which < 2) {
+    a.d_D.upload(dens, na * na, st);
+    dD = a.d_D.p;
+  }
+  const double* dC = nullptr;
+  if (mode == 1) {
+    const std::vector<double> row = coeff_row(coeff, (int)na, p);
+    a.d_cvec.upload(row.data(), na, st);
+    HIPCHK(hipStreamSynchronize(st));   // row is a stack-owned host buffer
+    dC = a.d_cvec.p;
+  }
+  lowdin_fock_stats stt{};
+  HIPCHK(hipEventRecord(c.ev0, st));
+  if (which < 2)
+    enqueue_store(a, dD, dC, dOut, mode, st, &stt);
+  else
+    enqueue_coupling_store(a, *b, dC, dOut, mode, st, &stt);
+  HIPCHK(hipEventRecord(c.ev1, st));
+  HIPCHK(hipMemcpyAsync(result, dOut, count * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  float kms = 0.f;
+  HIPCHK(hipEventElapsedTime(&kms, c.ev0, c.ev1));
+  stt.kernel_ms = kms;
+  stt.total_ms = now_ms() - t0;
+  a.stats = stt;
+}
+
 }  // namespace lowdin
 
 // ================================================================ C ABI
@@ -1062,6 +1267,99 @@ void LibintInterface_compute_coupling_direct(void* lint, void* olint, double* de
     lowdin::g_last_error = std::string("compute_coupling_direct: ") + e.what();
     fprintf(stderr, "lowdin_fock: %s\n", lowdin::g_last_error.c_str());
     lowdin::fill_nan(result, (size_t)s->nbasis * s->nbasis);
+  }
+}
+
+// Libint2Iface.cpp:3736-3750: every permutationally unique integral of the
+// Fock build's quartet set, normalised, into results[i n^3 + j n^2 + k n + l]
+// (all eight images; elements below the drop keep the caller's values).
+void LibintInterface_compute_2body_directAll(void* lint, double* dens, double* results) {
+  Species* s = static_cast<Species*>(lint);
+  try {
+    lowdin::run_store(*s, nullptr, 0, dens, nullptr, 0, results);
+    lowdin::g_last_error.clear();
+  } catch (const std::exception& e) {
+    lowdin::set_last_error(std::string("compute_2body_directAll: ") + e.what());
+  }
+}
+
+// Libint2Iface.cpp:3528-3549: result[j n^2 + k n + l] = sum_i C(p,i) (ij|kl),
+// C(p,i) = coeff[p + i n].
+void LibintInterface_compute_2body_directIT(void* lint, double* dens, double* coeff, double* result, int* p) {
+  Species* s = static_cast<Species*>(lint);
+  try {
+    lowdin::run_store(*s, nullptr, 1, dens, coeff, *p, result);
+    lowdin::g_last_error.clear();
+  } catch (const std::exception& e) {
+    lowdin::set_last_error(std::string("compute_2body_directIT: ") + e.what());
+    const size_t n = s->nbasis;
+    lowdin::fill_nan(result, n * n * n);
+  }
+}
+
+// Libint2Iface.cpp:3752-3773 (dens is not read, as in the reference).
+void LibintInterface_compute_coupling_directAll(void* lint, void* olint, double* dens, double* results) {
+  (void)dens;
+  Species* s = static_cast<Species*>(lint);
+  try {
+    lowdin::run_store(*s, static_cast<Species*>(olint), 2, nullptr, nullptr, 0, results);
+    lowdin::g_last_error.clear();
+  } catch (const std::exception& e) {
+    lowdin::set_last_error(std::string("compute_coupling_directAll: ") + e.what());
+  }
+}
+
+// Libint2Iface.cpp:3591-3623: results[v n_o^2 + l n_o + s] = sum_u C(p,u)
+// (uv|ls), u, v of lint's species (dens is not read, as in the reference).
+void LibintInterface_compute_coupling_directIT(void* lint, void* olint, double* dens, double* coeff, double* results,
+                                               int* p) {
+  (void)dens;
+  Species* s = static_cast<Species*>(lint);
+  Species* o = static_cast<Species*>(olint);
+  try {
+    lowdin::run_store(*s, o, 3, nullptr, coeff, *p, results);
+    lowdin::g_last_error.clear();
+  } catch (const std::exception& e) {
+    lowdin::set_last_error(std::string("compute_coupling_directIT: ") + e.what());
+    const size_t n = s->nbasis, on = o->nbasis;
+    lowdin::fill_nan(results, n * on * on);
+  }
+}
+
+// Device-resident forms of the four calls above (device pointers, work on
+// the library's stream, synchronised before return).
+int lowdin_fock_store_device(void* lint, void* olint, int which, const double* d_dens, const double* d_crow,
+                             double* d_out) {
+  Species* s = static_cast<Species*>(lint);
+  Species* o = static_cast<Species*>(olint);
+  lowdin::Context& c = lowdin::ctx();
+  try {
+    if (which < 0 || which > 3) throw std::runtime_error("which must be 0..3");
+    if (which >= 2 && !o) throw std::runtime_error("coupling variants need the other species");
+    if ((which & 1) && !d_crow) throw std::runtime_error("transform needs the coefficient row");
+    if (which < 2 && s->is_electron && !d_dens) throw std::runtime_error("screened store needs the density");
+    std::lock_guard<std::mutex> lk(c.mu);
+    const double t0 = lowdin::now_ms();
+    lowdin::ensure_context();
+    hipStream_t st = c.stream;
+    lowdin_fock_stats stt{};
+    HIPCHK(hipEventRecord(c.ev0, st));
+    if (which < 2)
+      lowdin::enqueue_store(*s, d_dens, d_crow, d_out, which & 1, st, &stt);
+    else
+      lowdin::enqueue_coupling_store(*s, *o, d_crow, d_out, which & 1, st, &stt);
+    HIPCHK(hipEventRecord(c.ev1, st));
+    HIPCHK(hipStreamSynchronize(st));
+    float kms = 0.f;
+    HIPCHK(hipEventElapsedTime(&kms, c.ev0, c.ev1));
+    stt.kernel_ms = kms;
+    stt.total_ms = lowdin::now_ms() - t0;
+    s->stats = stt;
+    lowdin::g_last_error.clear();
+    return 0;
+  } catch (const std::exception& e) {
+    lowdin::set_last_error(std::string("store_device: ") + e.what());
+    return -1;
   }
 }
 

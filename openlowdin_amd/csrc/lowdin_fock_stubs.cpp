@@ -2,9 +2,9 @@
 //
 // openLOWDIN's Fortran interface block (src/ints/Libint2Interface.f90:165-373)
 // binds every entry point of src/ints/Libint2Iface.h:350-453.  The direct
-// Fock path needs only the ones implemented in lowdin_fock.cpp; the disk,
-// integral-transformation, all-integral and G12 variants below are outside
-// this library's scope.  They exist so that a Fortran build of openLOWDIN
+// Fock path and its integral-store / first-quarter-transform variants are
+// implemented in lowdin_fock.cpp; the disk, alpha-beta and G12 variants
+// below are outside this library's scope.  They exist so that a Fortran build of openLOWDIN
 // links against liblowdin_fock.so unchanged; calling one records an error
 // (lowdin_fock_last_error) and prints it, leaving outputs untouched.
 #include <cstdio>
@@ -18,14 +18,8 @@ void set_unsupported(const char* name);
 
 extern "C" {
 
-void LibintInterface_compute_2body_directIT(void*, double*, double*, double*, int*) {
-  LOWDIN_UNSUPPORTED("LibintInterface_compute_2body_directIT");
-}
 void LibintInterface_compute_2body_disk(void*, const char*, double*) {
   LOWDIN_UNSUPPORTED("LibintInterface_compute_2body_disk");
-}
-void LibintInterface_compute_coupling_directIT(void*, void*, double*, double*, double*, int*) {
-  LOWDIN_UNSUPPORTED("LibintInterface_compute_coupling_directIT");
 }
 void LibintInterface_compute_alphabeta_direct(void*, void*, double*, double*, double*) {
   LOWDIN_UNSUPPORTED("LibintInterface_compute_alphabeta_direct");
@@ -45,12 +39,6 @@ void libintinterface_compute_g12_direct(void*, double*, double*, double*, const 
 void libintinterface_compute_g12inter_direct(void*, void*, double*, double*, const double*, const double*,
                                              const int) {
   LOWDIN_UNSUPPORTED("libintinterface_compute_g12inter_direct");
-}
-void LibintInterface_compute_2body_directAll(void*, double*, double*) {
-  LOWDIN_UNSUPPORTED("LibintInterface_compute_2body_directAll");
-}
-void LibintInterface_compute_coupling_directAll(void*, void*, double*, double*) {
-  LOWDIN_UNSUPPORTED("LibintInterface_compute_coupling_directAll");
 }
 void libintinterface_compute_g12_directAll(void*, double*, double*, const double*, const double*, const int) {
   LOWDIN_UNSUPPORTED("libintinterface_compute_g12_directAll");

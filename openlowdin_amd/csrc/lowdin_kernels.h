@@ -69,6 +69,59 @@ struct CouplingArgs {
   double drop;             // raw-integral drop (1e-10)
 };
 
+// Integral store (mode 0) or first-quarter transform (mode 1) over the
+// intra-species quartets of one species.
+//   mode 0: A[i][j][k][l] (n^4) = (ij|kl) * norma^4 for |raw| > drop, all
+//           eight index images; other elements are left as they are.
+//   mode 1: GG[j][k][l] (n^3) += sum_i c[i] (ij|kl), k <= l only (the other
+//           triangle is mirrored afterwards); GG must start at zero.
+struct StoreArgs {
+  const Task* tasks;
+  int ntasks;
+  const PairInfo* bra;     // pairs of the bra class list
+  const PairInfo* ket;     // pairs of the ket class list
+  const double* prim;
+  const double* norma;
+  const double* Dn;        // nshell x nshell block norms of D (screen only)
+  const double* c;         // mode 1: row p of the coefficient matrix (n)
+  double* out;
+  const double* boys;
+  int n;
+  int nshell;
+  int same_class;          // bra and ket lists are the same class list
+  int screen;              // Dnorm * Q * Q < eps screen (electrons)
+  int mode;
+  int pad;
+  double eps;
+  double drop;             // kept if |raw| > drop
+};
+
+// The same over the quartets (pair of species X | pair of species Y) of an
+// inter-species pair, X the species whose shells run the outer loops.
+//   mode 0: A[x1][x2][y1][y2] = (x1x2|y1y2) * norma^4 for |raw| >= drop,
+//           strides (sx1, sx2, sy1, 1); four images (x1 <-> x2, y1 <-> y2).
+//   mode 1: GG[v][l][s] (n_T x n_O x n_O) += sum_u c[u] (uv|ls), u, v of
+//           the transformed species T, l <= s of the other species O.
+struct CouplingStoreArgs {
+  const Task* tasks;
+  int ntasks;
+  const PairInfo* bra;
+  const PairInfo* ket;
+  const double* bprim;
+  const double* kprim;
+  const double* nbra_norma;
+  const double* nket_norma;
+  const double* c;         // mode 1: row p of T's coefficient matrix
+  double* out;
+  const double* boys;
+  int bra_is_x;            // mode 0: bra pair belongs to X
+  int bra_is_t;            // mode 1: bra pair belongs to T
+  int mode;
+  int n_o;                 // mode 1: basis size of O
+  size_t sx1, sx2, sy1;    // mode 0 strides
+  double drop;             // dropped if |raw| < drop
+};
+
 // Raw overlap block <A|B> (row-major, A's components slowest) of two shells
 // with l <= 2, from primitives whose coefficients already carry N(alpha,l).
 // Obara-Saika 1-D recursion S(i+1,j) = PA S(i,j) + (i S(i-1,j) + j S(i,j-1))/(2p).
@@ -137,6 +190,8 @@ void launch_absmax(const double* x, size_t n, double* out, hipStream_t s);
 void launch_scale_cols(double* X, const double* U, const double* w, int n, int m, int col0, hipStream_t s);
 void launch_symmetrize_batch(const SymDesc* d, int nmol, int nmax, hipStream_t s);
 void launch_fill_zero(double* p, size_t count, hipStream_t s);
+// GG[i][k][j] = GG[i][j][k] for j < k, i < n1, j, k < n2.
+void launch_mirror_upper(double* GG, int n1, int n2, hipStream_t s);
 void launch_dnorm_batch(const MolDesc* mols, int nmol, int nshmax, hipStream_t s);
 void launch_reduce_counts(const unsigned int* counts, int ntasks, unsigned long long* out, hipStream_t s);  // 3 per task
 void launch_reduce_counts_multi(const unsigned int* counts, const int* ranges, int nlaunch, unsigned long long* out,

@@ -250,6 +250,18 @@ __global__ void fill_zero_kernel(double* p, size_t count) {
   for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += stride) p[k] = 0.0;
 }
 
+// Lower triangle of every (k, l) slice from the upper one: the
+// first-quarter-transform kernels accumulate k <= l only.
+__global__ void mirror_upper_kernel(double* GG, int n1, int n2) {
+  const size_t nn = (size_t)n2 * n2, total = (size_t)n1 * nn;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+    const size_t i = e / nn, r = e - i * nn;
+    const int j = (int)(r / n2), k = (int)(r - (size_t)j * n2);
+    if (k < j) GG[e] = GG[i * nn + (size_t)k * n2 + j];
+  }
+}
+
 // ------------------------------------------------------------ SCF helpers
 // F = H + qq G (+ Cs): the Fock matrix of a species from its core
 // Hamiltonian, two-particle matrix (scaled by the charge squared) and the
@@ -431,6 +443,11 @@ void launch_fill_zero(double* p, size_t count, hipStream_t s) {
 }
 
 static unsigned grid_for(size_t count) { return (unsigned)std::min<size_t>((count + 255) / 256, 4096); }
+
+void launch_mirror_upper(double* GG, int n1, int n2, hipStream_t s) {
+  const size_t total = (size_t)n1 * n2 * n2;
+  if (total) hipLaunchKernelGGL(mirror_upper_kernel, dim3(grid_for(total)), dim3(256), 0, s, GG, n1, n2);
+}
 
 void launch_fock_assemble(double* F, const double* H, const double* G, double qq, const double* Cs, size_t nn,
                           hipStream_t s) {

@@ -67,6 +67,12 @@ def lib():
         L.LibintInterface_init_2body_ints.argtypes = [vp]
         L.LibintInterface_compute_2body_direct.argtypes = [vp, dp, dp, dp]
         L.LibintInterface_compute_coupling_direct.argtypes = [vp, vp, dp, dp]
+        L.LibintInterface_compute_2body_directAll.argtypes = [vp, dp, dp]
+        L.LibintInterface_compute_2body_directIT.argtypes = [vp, dp, dp, dp, ip]
+        L.LibintInterface_compute_coupling_directAll.argtypes = [vp, vp, dp, dp]
+        L.LibintInterface_compute_coupling_directIT.argtypes = [vp, vp, dp, dp, dp, ip]
+        L.lowdin_fock_store_device.argtypes = [vp, vp, C.c_int, vp, vp, vp]
+        L.lowdin_fock_store_device.restype = C.c_int
         L.lowdin_fock_last_error.restype = C.c_char_p
         L.lowdin_fock_get_stats.argtypes = [vp, C.POINTER(Stats)]
         L.lowdin_fock_shellpair_list.argtypes = [vp, ip, ip, C.c_int]
@@ -214,6 +220,79 @@ class Libint2Interface:
         _check("compute_coupling_direct")
         return B
 
+    # Libint2Interface_compute2BodyIntraspecies_direct_all, Coulomb branch
+    # (Libint2Interface.f90:1065-1115 -> Libint2Iface.cpp:3736-3750)
+    def compute_2body_intraspecies_direct_all(self, density: np.ndarray, ints: np.ndarray = None) -> np.ndarray:
+        """Every integral of the Fock build's quartet set (the pair list of
+        the last init_2body_ints, the density-block screen for electrons),
+        |raw| > 1e-10, normalised, all eight images.  Returns the
+        (n, n, n, n) tensor, (i j|k l) at [i, j, k, l]; elements not written
+        keep the values of ``ints`` (zeros when None)."""
+        n = self.nbasis
+        D = _fortran(density, n)
+        A = np.zeros(n ** 4) if ints is None else np.array(ints, dtype=np.float64).reshape(-1).copy()
+        lib().LibintInterface_compute_2body_directAll(self.this, _dptr(D), _dptr(A))
+        _check("compute_2body_directAll")
+        return A.reshape(n, n, n, n)
+
+    # Libint2Interface_compute2BodyIntraspecies_direct_IT
+    # (Libint2Interface.f90:551-579 -> Libint2Iface.cpp:3528-3549)
+    def compute_2body_intraspecies_direct_IT(self, density: np.ndarray, coefficients: np.ndarray,
+                                             p: int) -> np.ndarray:
+        """First-quarter transform on orbital row p: GG[j, k, l] =
+        sum_i C(p, i) (i j|k l), C(p, i) = coefficients[p, i] (n x n)."""
+        n = self.nbasis
+        D = _fortran(density, n)
+        Cm = _fortran(coefficients, n)
+        GG = np.zeros(n ** 3)
+        pp = C.c_int(int(p))
+        lib().LibintInterface_compute_2body_directIT(self.this, _dptr(D), _dptr(Cm), _dptr(GG), C.byref(pp))
+        _check("compute_2body_directIT")
+        return GG.reshape(n, n, n)
+
+    # Libint2Interface_compute2BodyInterspecies_direct_all, Coulomb branch
+    # (Libint2Interface.f90:1117-1179 -> Libint2Iface.cpp:3752-3773)
+    def compute_2body_interspecies_direct_all(self, other: "Libint2Interface", density: np.ndarray,
+                                              ints: np.ndarray = None) -> np.ndarray:
+        """(x1 x2|y1 y2) over every pair of both species, x of the lower
+        species id: the (n_x, n_x, n_y, n_y) tensor with the reference's
+        strides (n n_o^2, n_o^2, n_o, 1), n = this species' basis size."""
+        n, no = self.nbasis, other.nbasis
+        D = np.asfortranarray(density, dtype=np.float64)   # passed through, not read
+        A = np.zeros(n * n * no * no) if ints is None else np.array(ints, dtype=np.float64).reshape(-1).copy()
+        lib().LibintInterface_compute_coupling_directAll(self.this, other.this, _dptr(D), _dptr(A))
+        _check("compute_coupling_directAll")
+        if self.species_id > other.species_id:
+            return A.reshape(no, no, n, n)
+        return A.reshape(n, n, no, no)
+
+    # Libint2Interface_compute2BodyInterspecies_direct_IT
+    # (Libint2Interface.f90:677-707 -> Libint2Iface.cpp:3591-3623)
+    def compute_2body_interspecies_direct_IT(self, other: "Libint2Interface", density: np.ndarray,
+                                             coefficients: np.ndarray, p: int) -> np.ndarray:
+        """GG[v, l, s] (n x n_o x n_o) = sum_u C(p, u) (u v|l s), u, v of this
+        species, l, s of the other."""
+        n, no = self.nbasis, other.nbasis
+        D = np.asfortranarray(density, dtype=np.float64)   # passed through, not read
+        Cm = _fortran(coefficients, n)
+        GG = np.zeros(n * no * no)
+        pp = C.c_int(int(p))
+        lib().LibintInterface_compute_coupling_directIT(self.this, other.this, _dptr(D), _dptr(Cm), _dptr(GG),
+                                                        C.byref(pp))
+        _check("compute_coupling_directIT")
+        return GG.reshape(n, no, no)
+
+    def store_device(self, which: int, d_out: int, d_dens: int = 0, d_crow: int = 0,
+                     other: "Libint2Interface" = None):
+        """Device-resident store / transform (which 0: intra all, 1: intra
+        first quarter, 2: inter all, 3: inter first quarter); device
+        addresses, synchronised on return."""
+        rc = lib().lowdin_fock_store_device(self.this, other.this if other is not None else None, int(which),
+                                            d_dens or None, d_crow or None, d_out)
+        if rc != 0:
+            _check("store_device")
+            raise LowdinFockError("store_device failed")
+
     def coupling_pair(self, other: "Libint2Interface", d_self: np.ndarray, d_other: np.ndarray):
         """Both coupling directions from one pass over the quartets."""
         Da = _fortran(d_self, self.nbasis)
@@ -305,8 +384,11 @@ class Libint2Interface:
 
 # ------------------------------------------------ DirectIntegralManager
 def _instance(objs: Dict[int, Libint2Interface], system, species_id: int) -> Libint2Interface:
+    """A species' handle, constructed with its pair list on first use
+    (DirectIntegralManager_constructor, DirectIntegralManager.f90:457-460)."""
     if species_id not in objs or not objs[species_id].is_instanced:
         objs[species_id] = Libint2Interface(system.species[species_id - 1], species_id)
+        objs[species_id].init_2body_ints()
     return objs[species_id]
 
 
@@ -327,3 +409,40 @@ def get_direct_inter_repulsion_matrix(species_id: int, other_species_id: int, sc
     a = _instance(objs, system, species_id)
     b = _instance(objs, system, other_species_id)
     return a.compute_2body_interspecies_direct(b, np.array(density))
+
+
+def get_direct_intra_repulsion_first_quarter(species_id: int, scheme: str, density: np.ndarray,
+                                             coefficients: np.ndarray, p: int, system,
+                                             objs: Dict[int, Libint2Interface]) -> np.ndarray:
+    """DirectIntegralManager_getDirectIntraRepulsionFirstQuarter
+    (DirectIntegralManager.f90:274-327): the (n, n, n) first-quarter
+    transform on coefficient row p.  ``scheme`` is not consulted, as in the
+    reference (its select case is commented out)."""
+    return _instance(objs, system, species_id).compute_2body_intraspecies_direct_IT(
+        np.array(density), np.array(coefficients), p)
+
+
+def get_direct_inter_repulsion_first_quarter(species_id: int, other_species_id: int, scheme: str,
+                                             density: np.ndarray, coefficients: np.ndarray, p: int, system,
+                                             objs: Dict[int, Libint2Interface]) -> np.ndarray:
+    """DirectIntegralManager_getDirectInterRepulsionFirstQuarter
+    (DirectIntegralManager.f90:336-391)."""
+    a = _instance(objs, system, species_id)
+    b = _instance(objs, system, other_species_id)
+    return a.compute_2body_interspecies_direct_IT(b, np.array(density), np.array(coefficients), p)
+
+
+def get_direct_intra_repulsion_integrals_all(species_id: int, density: np.ndarray, system,
+                                             objs: Dict[int, Libint2Interface], ints: np.ndarray = None):
+    """DirectIntegralManager_getDirectIntraRepulsionIntegralsAll
+    (DirectIntegralManager.f90:957-990): the species' (n, n, n, n) store."""
+    return _instance(objs, system, species_id).compute_2body_intraspecies_direct_all(np.array(density), ints)
+
+
+def get_direct_inter_repulsion_integrals_all(species_id: int, other_species_id: int, density: np.ndarray, system,
+                                             objs: Dict[int, Libint2Interface], ints: np.ndarray = None):
+    """DirectIntegralManager_getDirectInterRepulsionIntegralsAll
+    (DirectIntegralManager.f90:999-1034)."""
+    a = _instance(objs, system, species_id)
+    b = _instance(objs, system, other_species_id)
+    return a.compute_2body_interspecies_direct_all(b, np.array(density), ints)

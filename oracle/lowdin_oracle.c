@@ -29,6 +29,13 @@
  *     permuted/non-permuted scatter, no symmetrisation):
  *                                         src/ints/Libint2Iface.cpp:1111-1294
  *   - 1-body S/T/V with norma:            src/ints/Libint2Iface.cpp:163-217
+ *   - integral store over the Fock build's quartets (|raw| > 1e-10, eight
+ *     images, norma):                     src/ints/Libint2Iface.cpp:2552-2742
+ *   - inter-species integral store (all pairs, bf1 <= bf2 / obf1 <= obf2,
+ *     |raw| >= 1e-10, four images):       src/ints/Libint2Iface.cpp:2744-2908
+ *     (the first-quarter transforms of :620-921 and :1296-1622 are the
+ *     contraction of these tensors with one coefficient row, done in
+ *     oracle.py)
  *
  * Matrix layout: all n x n matrices are dense FP64 in the memory image the
  * reference's C wrappers produce (Libint2Iface.cpp:3510-3526, 3566-3589):
@@ -601,6 +608,130 @@ void orc_compute_coupling_direct(const orc_basis *outer, const orc_basis *inner,
   }
   if (counts) { counts[0] = kept; counts[1] = near; }
   free(Bt); free(normA); free(normB); free(off); free(ooff);
+}
+
+/* ------------------------------------------------ integral store */
+/* Reference LibintInterface::compute_2body_directAll
+ * (Libint2Iface.cpp:2552-2742): the quartet loops and screen of the direct
+ * build (2629-2669); each integral with |raw| > drop, times the four norma,
+ * written to A[i n^3 + j n^2 + k n + l] and its seven other index images
+ * (2701-2717).  Elements never written keep the caller's values.  The
+ * reference visits each unique integral once (IntraIntsIt); writing the
+ * images of every element of the shell block covers the same set.
+ * Returns the number of shell quartets that passed the screen. */
+long orc_compute_2body_all(const orc_basis *bs, const int *pairs_s1, const int *pairs_s2, int npairs,
+                           const double *Q, const double *Dmat, int screen, double precision, double drop,
+                           double *A) {
+  const int ns = bs->nshell, n = orc_nbasis(bs);
+  const size_t N = (size_t)n, N2 = N * N, N3 = N2 * N;
+  int *off = (int *)malloc(sizeof(int) * ns);
+  shell_offsets(bs, off);
+  double *norma = (double *)malloc(sizeof(double) * n);
+  orc_norma(bs, norma);
+  int *start = (int *)calloc(ns + 1, sizeof(int));
+  for (int k = 0; k < npairs; ++k) start[pairs_s1[k] + 1]++;
+  for (int s = 0; s < ns; ++s) start[s + 1] += start[s];
+  int *plist = (int *)malloc(sizeof(int) * (npairs > 0 ? npairs : 1));
+  {
+    int *fill = (int *)calloc(ns, sizeof(int));
+    for (int k = 0; k < npairs; ++k) plist[start[pairs_s1[k]] + fill[pairs_s1[k]]++] = pairs_s2[k];
+    free(fill);
+  }
+  double *Dn = (double *)calloc((size_t)ns * ns, sizeof(double));
+  for (int s1 = 0; s1 < ns; ++s1)
+    for (int s2 = 0; s2 < ns; ++s2) {
+      double m = 0.0;
+      for (int i = 0; i < ncart(bs->l[s1]); ++i)
+        for (int j = 0; j < ncart(bs->l[s2]); ++j) {
+          const double v = fabs(Dmat[(size_t)(off[s1] + i) * n + off[s2] + j]);
+          if (v > m) m = v;
+        }
+      Dn[s1 * ns + s2] = m;
+    }
+  double *buf = (double *)malloc(sizeof(double) * ORC_MAXNC * ORC_MAXNC * ORC_MAXNC * ORC_MAXNC);
+  long nquart = 0;
+  for (int s1 = 0; s1 < ns; ++s1) {
+    const int n1 = ncart(bs->l[s1]);
+    for (int k2 = start[s1]; k2 < start[s1 + 1]; ++k2) {
+      const int s2 = plist[k2], n2 = ncart(bs->l[s2]);
+      for (int s3 = 0; s3 <= s1; ++s3) {
+        const int n3 = ncart(bs->l[s3]);
+        const int s4max = (s1 == s3) ? s2 : s3;
+        for (int k4 = start[s3]; k4 < start[s3 + 1]; ++k4) {
+          const int s4 = plist[k4];
+          if (s4 > s4max) break;
+          if (screen) {
+            double d = Dn[s1 * ns + s2];
+            const double o[5] = {Dn[s1 * ns + s3], Dn[s2 * ns + s3], Dn[s1 * ns + s4], Dn[s2 * ns + s4],
+                                 Dn[s3 * ns + s4]};
+            for (int q = 0; q < 5; ++q) if (o[q] > d) d = o[q];
+            if (d * Q[s1 * ns + s2] * Q[s3 * ns + s4] < precision) continue;
+          }
+          const int n4 = ncart(bs->l[s4]);
+          ++nquart;
+          orc_eri_shell(bs, s1, s2, bs, s3, s4, 0.0, buf);
+          for (int f1 = 0, f = 0; f1 < n1; ++f1)
+            for (int f2 = 0; f2 < n2; ++f2)
+              for (int f3 = 0; f3 < n3; ++f3)
+                for (int f4 = 0; f4 < n4; ++f4, ++f) {
+                  if (!(fabs(buf[f]) > drop)) continue;
+                  const size_t i = off[s1] + f1, j = off[s2] + f2, k = off[s3] + f3, l = off[s4] + f4;
+                  const double v = buf[f] * norma[i] * norma[j] * norma[k] * norma[l];
+                  A[i * N3 + j * N2 + k * N + l] = v;
+                  A[j * N3 + i * N2 + k * N + l] = v;
+                  A[i * N3 + j * N2 + l * N + k] = v;
+                  A[j * N3 + i * N2 + l * N + k] = v;
+                  A[k * N3 + l * N2 + i * N + j] = v;
+                  A[k * N3 + l * N2 + j * N + i] = v;
+                  A[l * N3 + k * N2 + i * N + j] = v;
+                  A[l * N3 + k * N2 + j * N + i] = v;
+                }
+        }
+      }
+    }
+  }
+  free(buf); free(Dn); free(plist); free(start); free(norma); free(off);
+  return nquart;
+}
+
+/* Reference LibintInterface::compute_coupling_directAll
+ * (Libint2Iface.cpp:2744-2908) with `this` = outer: every s1 <= s2 of outer
+ * and os1 <= os2 of inner, no screen, elements bf1 <= bf2 and obf1 <= obf2
+ * with |raw| >= drop (2864-2866), normalised, written as
+ * T[bf1][bf2][obf1][obf2] (dims n_outer^2 x n_inner^2) and the images
+ * bf1 <-> bf2, obf1 <-> obf2 (2877-2880).  Unwritten elements keep the
+ * caller's values. */
+void orc_compute_coupling_all(const orc_basis *outer, const orc_basis *inner, double drop, double *T) {
+  const int ns = outer->nshell, ons = inner->nshell;
+  const int nA = orc_nbasis(outer), nB = orc_nbasis(inner);
+  const size_t s1x = (size_t)nA * nB * nB, s2x = (size_t)nB * nB, s3x = (size_t)nB;
+  int *off = (int *)malloc(sizeof(int) * ns), *ooff = (int *)malloc(sizeof(int) * ons);
+  shell_offsets(outer, off); shell_offsets(inner, ooff);
+  double *normA = (double *)malloc(sizeof(double) * nA), *normB = (double *)malloc(sizeof(double) * nB);
+  orc_norma(outer, normA); orc_norma(inner, normB);
+  double *buf = (double *)malloc(sizeof(double) * ORC_MAXNC * ORC_MAXNC * ORC_MAXNC * ORC_MAXNC);
+  for (int s1 = 0; s1 < ns; ++s1)
+    for (int s2 = s1; s2 < ns; ++s2)
+      for (int os1 = 0; os1 < ons; ++os1)
+        for (int os2 = os1; os2 < ons; ++os2) {
+          const int n1 = ncart(outer->l[s1]), n2 = ncart(outer->l[s2]);
+          const int on1 = ncart(inner->l[os1]), on2 = ncart(inner->l[os2]);
+          orc_eri_shell(outer, s1, s2, inner, os1, os2, 0.0, buf);
+          for (int f1 = 0, f = 0; f1 < n1; ++f1)
+            for (int f2 = 0; f2 < n2; ++f2)
+              for (int of1 = 0; of1 < on1; ++of1)
+                for (int of2 = 0; of2 < on2; ++of2, ++f) {
+                  const size_t b1 = off[s1] + f1, b2 = off[s2] + f2, o1 = ooff[os1] + of1, o2 = ooff[os2] + of2;
+                  if (!(b1 <= b2 && o1 <= o2)) continue;
+                  if (fabs(buf[f]) < drop) continue;
+                  const double v = buf[f] * normA[b1] * normA[b2] * normB[o1] * normB[o2];
+                  T[b1 * s1x + b2 * s2x + o1 * s3x + o2] = v;
+                  T[b2 * s1x + b1 * s2x + o1 * s3x + o2] = v;
+                  T[b1 * s1x + b2 * s2x + o2 * s3x + o1] = v;
+                  T[b2 * s1x + b1 * s2x + o2 * s3x + o1] = v;
+                }
+        }
+  free(buf); free(normA); free(normB); free(off); free(ooff);
 }
 
 /* ------------------------------------------------ 1-body (S, T, V) */

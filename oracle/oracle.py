@@ -56,6 +56,10 @@ def lib():
         L.orc_compute_coupling_direct.argtypes = [C.c_void_p, C.c_void_p, dp, C.c_int, C.c_double,
                                                   C.c_int, dp, lp]
         L.orc_1body.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, dp, dp]
+        L.orc_compute_2body_all.argtypes = [C.c_void_p, ip, ip, C.c_int, dp, dp, C.c_int, C.c_double,
+                                            C.c_double, dp]
+        L.orc_compute_2body_all.restype = C.c_long
+        L.orc_compute_coupling_all.argtypes = [C.c_void_p, C.c_void_p, C.c_double, dp]
         _LIB = L
     return _LIB
 
@@ -181,3 +185,73 @@ def one_body(b: Basis, kind: int, charges=()) -> np.ndarray:
     M = np.zeros(b.n * b.n)
     lib().orc_1body(b.ptr, kind, len(charges), Z, X, M)
     return M.reshape(b.n, b.n)
+
+
+def compute_2body_directAll(b: Basis, D: np.ndarray, pairs=None, Q=None, precision: float = DBL_EPSILON,
+                            drop: float = 1e-10, A: np.ndarray = None) -> np.ndarray:
+    """LibintInterface::compute_2body_directAll (Libint2Iface.cpp:2552-2742)
+    with the C wrapper's Schwarz matrix (3736-3750).  Returns the (n, n, n, n)
+    tensor, element (i j|k l) at [i, j, k, l] (memory i n^3 + j n^2 + k n + l);
+    elements the store does not write keep the values of ``A`` (zeros when
+    None).  ``drop`` = 0 keeps every integral of the screened quartets."""
+    if pairs is None:
+        pairs, _ = shellpair_list(b)
+    if Q is None:
+        Q = schwarz(b)
+    n = b.n
+    out = np.zeros(n ** 4) if A is None else np.array(A, dtype=np.float64).reshape(-1).copy()
+    s1 = np.ascontiguousarray(pairs[:, 0], dtype=np.int32)
+    s2 = np.ascontiguousarray(pairs[:, 1], dtype=np.int32)
+    screen = 1 if b.species.is_electron else 0
+    lib().orc_compute_2body_all(b.ptr, s1, s2, len(s1), np.ascontiguousarray(Q, dtype=np.float64).reshape(-1),
+                                np.ascontiguousarray(D, dtype=np.float64).reshape(-1), screen, precision,
+                                float(drop), out)
+    return out.reshape(n, n, n, n)
+
+
+def compute_2body_directIT(b: Basis, D: np.ndarray, Cmat: np.ndarray, p: int, **kw) -> np.ndarray:
+    """LibintInterface::compute_2body_directIT (Libint2Iface.cpp:620-921):
+    GG[j, k, l] = sum_i C(p, i) (i j|k l) over the quartets, screen and drop
+    of compute_2body_directAll, with C(p, i) = Cmat[p, i] (the Eigen copy of
+    the Fortran coefficient array, :3535-3537).  The reference accumulates
+    each unique integral into its distinct index permutations and mirrors
+    the k > l triangle (:776-875); the sum over the full store tensor is the
+    same quantity."""
+    A = compute_2body_directAll(b, D, **kw)
+    return np.tensordot(np.asarray(Cmat, dtype=np.float64)[p, :], A, axes=(0, 0))
+
+
+def _coupling_tensor(outer: Basis, inner: Basis, drop: float) -> np.ndarray:
+    T = np.zeros(outer.n * outer.n * inner.n * inner.n)
+    lib().orc_compute_coupling_all(outer.ptr, inner.ptr, float(drop), T)
+    return T.reshape(outer.n, outer.n, inner.n, inner.n)
+
+
+def compute_coupling_directAll(this: Basis, other: Basis, id_this: int, id_other: int,
+                               drop: float = 1e-10) -> np.ndarray:
+    """LibintInterface_compute_coupling_directAll (Libint2Iface.cpp:3752-3773,
+    2744-2908): the lower species id runs the outer loops; the result is
+    (n_outer, n_outer, n_inner, n_inner) with the reference's strides
+    (n n_o^2, n_o^2, n_o, 1), n = this.n, n_o = other.n.  Called from the
+    higher species id those strides only fit when n == n_o (ValueError
+    otherwise)."""
+    permuted = id_this > id_other
+    if permuted and this.n != other.n:
+        raise ValueError("reference layout undefined for a permuted call with different basis sizes")
+    outer, inner = (other, this) if permuted else (this, other)
+    return _coupling_tensor(outer, inner, drop)
+
+
+def compute_coupling_directIT(this: Basis, other: Basis, id_this: int, id_other: int, Cmat: np.ndarray,
+                              p: int, drop: float = 1e-10) -> np.ndarray:
+    """LibintInterface::compute_coupling_directIT (Libint2Iface.cpp:1296-1622):
+    GG[v, l, s] (n_this x n_other x n_other) = sum_u C(p, u) (u v|l s), u, v
+    of ``this``, l, s of ``other``, over every pair of both species with the
+    |raw| >= drop element sets of compute_coupling_directAll."""
+    permuted = id_this > id_other
+    c = np.asarray(Cmat, dtype=np.float64)[p, :]
+    if permuted:
+        T = _coupling_tensor(other, this, drop)          # [l, s, u, v]
+        return np.einsum("u,lsuv->vls", c, T)
+    T = _coupling_tensor(this, other, drop)              # [u, v, l, s]
+    return np.tensordot(c, T, axes=(0, 0))
