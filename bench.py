@@ -233,6 +233,71 @@ def serving_bench(S, SV, njobs, rel_rates, mixed_tau, seed=20261202):
     return out
 
 
+def store_bench(H, LI, S, steps, D510, h510, eris510, cpu=True):
+    """Integral store and first-quarter transform (the MEMORY / integral-
+    transformation variants): device-resident calls, each synchronised.
+      transform: C20H42 cc-pVDZ (n=510), the Fock build's quartets and
+        screen, GG = n^3 = 1.06 GB, orbital row 0 of a random C;
+      store: water hexamer 6-31G** (n=150), A = n^4 = 4.05 GB;
+      CPU: the oracle's transform on the water tetramer (n=100) beside the
+        GPU's on the same input."""
+    out = {}
+    n = h510.nbasis
+    rng = np.random.default_rng(31)
+    dD = H.DeviceArray.from_host(np.asfortranarray(D510).ravel(order="F"))
+    dc = H.DeviceArray.from_host(rng.normal(size=n) / np.sqrt(n))
+    dGG = H.DeviceArray((n ** 3,))
+    h510.store_device(1, dGG.ptr, dD.ptr, dc.ptr)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        h510.store_device(1, dGG.ptr, dD.ptr, dc.ptr)
+    dt = (time.perf_counter() - t0) / steps
+    st = h510.stats()
+    out["transform_c20h42"] = {"ms_per_transform": round(dt * 1e3, 3), "kernel_ms": round(st["kernel_ms"], 3),
+                               "eris_per_s": round(eris510 / dt, 1), "output_bytes": n ** 3 * 8}
+    dGG.free()
+    dD.free()
+    sp = S.water_cluster(6, np.random.default_rng(2)).species[0]
+    m = sp.nbasis
+    h = LI.Libint2Interface(sp, 1)
+    h.init_2body_ints()
+    Dm = synthetic_density(m, sp.n_occupied, seed=5)
+    dDm = H.DeviceArray.from_host(np.ascontiguousarray(Dm))
+    dA = H.DeviceArray.from_host(np.zeros(m ** 4))
+    h.store_device(0, dA.ptr, dDm.ptr)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        h.store_device(0, dA.ptr, dDm.ptr)
+    dt = (time.perf_counter() - t0) / steps
+    st = h.stats()
+    out["store_water6"] = {"n": m, "ms_per_store": round(dt * 1e3, 3), "kernel_ms": round(st["kernel_ms"], 3),
+                           "candidate_quartets": st["quartets"], "output_bytes": m ** 4 * 8}
+    dA.free()
+    h.close()
+    if cpu:
+        sys.path.insert(0, os.path.join(HERE, "oracle"))
+        import oracle as O
+        sp = S.water_cluster(4, np.random.default_rng(2)).species[0]
+        m = sp.nbasis
+        Dm = synthetic_density(m, sp.n_occupied, seed=5)
+        Cm = np.random.default_rng(6).normal(size=(m, m)) / np.sqrt(m)
+        h = LI.Libint2Interface(sp, 1)
+        h.init_2body_ints()
+        h.compute_2body_intraspecies_direct_IT(Dm, Cm, 0)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            G = h.compute_2body_intraspecies_direct_IT(Dm, Cm, 0)
+        tg = (time.perf_counter() - t0) / steps
+        t0 = time.perf_counter()
+        Gr = O.compute_2body_directIT(O.Basis(sp), Dm, Cm, 0)
+        tc = time.perf_counter() - t0
+        out["transform_water4_vs_cpu"] = {"n": m, "gpu_ms_host_buffers": round(tg * 1e3, 3),
+                                          "cpu_ms": round(tc * 1e3, 1), "cores": 1, "kind": "port",
+                                          "max_abs_diff": float(np.max(np.abs(G - Gr)))}
+        h.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -247,6 +312,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the served-SCF runs")
     ap.add_argument("--serving-jobs", type=int, default=16)
     ap.add_argument("--serving-rates", default="0.5,1,2")
+    ap.add_argument("--no-store", action="store_true", help="skip the integral store / transform runs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 256 MiB device-to-device copy (profiler byte-counter calibration)")
     args = ap.parse_args()
@@ -383,6 +449,10 @@ def main():
             print("[bench] served SCF...", file=sys.stderr, flush=True)
             out["serving"] = serving_bench(S, SV, args.serving_jobs,
                                            [float(r) for r in args.serving_rates.split(",")], tau_best)
+    if not args.no_store:
+        print("[bench] integral store / first-quarter transform...", file=sys.stderr, flush=True)
+        out["store"] = store_bench(H, LI, S, max(2, min(args.steps, 5)), D, h, st["eris"],
+                                   cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle sample)...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(sp, D, args.cpu_stride, st["eris"])

@@ -111,6 +111,25 @@ void LibintInterface_compute_coupling_directAll(void* lint, void* olint, double*
 void LibintInterface_compute_coupling_directIT(void* lint, void* olint, double* dens, double* coeff, double* results,
                                                int* p);
 
+/* Libint2Iface.cpp:3687-3706 (libintinterface_compute_g12_direct,
+ * compute_g12_direct at :2193-2365).  result (n x n) = J[D] + factor K[D]
+ * with the contracted Gaussian geminal sum_k coefficients[k]
+ * exp(-exponents[k] r12^2) (k < pot_size <= 64, exponents >= 0) in place of
+ * 1/r12: every quartet of the pair list (no density screen), degeneracy,
+ * G = 0.25 (G + G^T).  factor by address, pot_size by value, as bound by
+ * Libint2Interface.f90:305-316. */
+void libintinterface_compute_g12_direct(void* lint, double* dens, double* result, double* factor,
+                                        const double* coefficients, const double* exponents, const int pot_size);
+
+/* Libint2Iface.cpp:3708-3734 (libintinterface_compute_g12inter_direct,
+ * compute_g12inter_direct at :2367-2550).  result (n_lint x n_lint) = the
+ * geminal coupling on lint's species from the density `dens` (n_olint x
+ * n_olint) of olint's species, with the quartets, |raw| < 1e-10 drop and
+ * layout of LibintInterface_compute_coupling_direct. */
+void libintinterface_compute_g12inter_direct(void* lint, void* olint, double* dens, double* result,
+                                             const double* coefficients, const double* exponents,
+                                             const int pot_size);
+
 /* ------------------------------------------------ extensions (no reference
  * counterpart; used by this package's host code, tests and benchmarks) */
 
@@ -168,6 +187,14 @@ int lowdin_fock_coupling_pair_device(void* a, void* b, const double* d_da, const
  * quartets before the density screen.  Returns 0 on success. */
 int lowdin_fock_store_device(void* lint, void* olint, int which, const double* d_dens, const double* d_crow,
                              double* d_out);
+
+/* Device-resident G12 builds: olint NULL = libintinterface_compute_g12_direct
+ * on lint (d_dens, d_result n x n; factor), otherwise
+ * libintinterface_compute_g12inter_direct (d_dens n_olint^2, d_result
+ * n_lint^2).  Geminal parameters are host arrays.  Runs on the library
+ * stream and synchronises.  Returns 0 on success. */
+int lowdin_fock_g12_device(void* lint, void* olint, const double* d_dens, double* d_result, double factor,
+                           const double* coefficients, const double* exponents, int pot_size);
 
 /* Device-resident variant of LibintInterface_compute_2body_direct: d_dens
  * and d_result are device pointers (n x n FP64); the build is enqueued on

@@ -656,21 +656,12 @@ __global__ __launch_bounds__(256) void jk_strip_kernel(JKArgs a) {
 // s1 <= s2 of both species.  Computes the result on the bra species from the
 // ket species' density (rbra) and/or on the ket species from the bra
 // species' density (rket), each with the reference's element sets and drop.
+// Drop, density weights and scatter of one coupling quartet's raw block
+// (shared by the Coulomb and the Gaussian-geminal coupling kernels).
 template <int LA, int LB, int LC, int LD>
-__global__ __launch_bounds__(256) void coupling_kernel(CouplingArgs a) {
+__device__ __forceinline__ void coupling_scatter(const CouplingArgs& a, const PairInfo& P, const PairInfo& K,
+                                                 double* out) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
-  const int wid = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
-  if (wid >= a.ntasks) return;
-  const Task t = a.tasks[wid];
-  const int lane = threadIdx.x & 63;
-  const int j = t.j0 + lane;
-  if (j >= t.j1) return;
-  const PairInfo& P = a.bra[t.bra];
-  const PairInfo K = a.ket[j];
-  double out[NA * NB * NC * ND];
-  contracted_quartet<LA, LB, LC, LD>(a.bprim + (size_t)P.poff * kPrimPairStride, P.nprim,
-                                     a.kprim + (size_t)K.poff * kPrimPairStride, K.nprim, a.boys,
-                                     P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
   const double drop = a.drop;
 #pragma unroll
   for (int k = 0; k < NA * NB * NC * ND; ++k)
@@ -728,6 +719,24 @@ __global__ __launch_bounds__(256) void coupling_kernel(CouplingArgs a) {
         unsafeAtomicAdd(&a.rket[(size_t)r * a.nket + c], s * a.nket_norma[fc] * a.nket_norma[fd]);
       }
   }
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(256) void coupling_kernel(CouplingArgs a) {
+  constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  const int wid = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  if (wid >= a.ntasks) return;
+  const Task t = a.tasks[wid];
+  const int lane = threadIdx.x & 63;
+  const int j = t.j0 + lane;
+  if (j >= t.j1) return;
+  const PairInfo& P = a.bra[t.bra];
+  const PairInfo K = a.ket[j];
+  double out[NA * NB * NC * ND];
+  contracted_quartet<LA, LB, LC, LD>(a.bprim + (size_t)P.poff * kPrimPairStride, P.nprim,
+                                     a.kprim + (size_t)K.poff * kPrimPairStride, K.nprim, a.boys,
+                                     P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
+  coupling_scatter<LA, LB, LC, LD>(a, P, K, out);
 }
 
 // --------------------------------------------------- Schwarz (a11) kernel

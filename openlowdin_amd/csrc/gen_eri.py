@@ -2,10 +2,12 @@
 
 For every canonical quartet class (la lb|lc ld) with l <= LMAX, la >= lb,
 lc >= ld and pair class (la, lb) >= (lc, ld), this emits two device
-functions into ``gen/eri_<class>.inc``, two kernel translation units per
-class (``gen/k_<class>.hip``: Fock build, coupling and Schwarz kernels;
-``gen/s_<class>.hip``: integral store and first-quarter transform), so that
-they compile in parallel, and the launch dispatch table ``gen/dispatch.h``:
+functions into ``gen/eri_<class>.inc`` (plus the Gaussian-geminal VRR
+``g12vrr_<class>``, see G12Gen), three kernel translation units per class
+(``gen/k_<class>.hip``: Fock build, coupling and Schwarz kernels;
+``gen/s_<class>.hip``: integral store and first-quarter transform;
+``gen/g_<class>.hip``: G12 Fock build and coupling), so that they compile in
+parallel, and the launch dispatch table ``gen/dispatch.h``:
 
 ``vrr_<la><lb><lc><ld>(F, ...)``
     Obara-Saika vertical recursion (Obara & Saika, J. Chem. Phys. 84, 3963
@@ -146,6 +148,64 @@ class VRRGen:
         return "\n".join([sig, unused] + self.lines + body + ["}"]), L
 
 
+class G12Gen:
+    """Primitive [e0|exp(-g r12^2)|f0] for one Gaussian-geminal term.
+
+    The integrand is a Gaussian in (x1, x2) per Cartesian axis, so each
+    integral is a product of three one-dimensional moments
+    I(i, k) = <(x1 - A)^i (x2 - C)^k> of that Gaussian.  With mean
+    (mu1, mu2) and covariance S = (2 M)^-1, M = [[p + g, -g], [-g, q + g]],
+    Gaussian integration by parts gives
+        I(i+1, k) = (mu1 - A) I(i, k) + i S11 I(i-1, k) + k S12 I(i, k-1)
+        I(i, k+1) = (mu2 - C) I(i, k) + i S12 I(i-1, k) + k S22 I(i, k-1)
+    with S11 = (q+g)/(2D), S12 = g/(2D), S22 = (p+g)/(2D), D = pq + g(p+q),
+    mu1 - A = PA - (g q / D) PQ, mu2 - C = QC + (g p / D) PQ.  The caller
+    supplies these and the prefactor; ``acc`` has the layout of the Coulomb
+    VRR's, so the same HRR finishes the quartet."""
+
+    def __init__(self, la, lb, lc, ld):
+        self.la, self.lb, self.lc, self.ld = la, lb, lc, ld
+        self.es = [c for l in range(la, la + lb + 1) for c in comps(l)]
+        self.fs = [c for l in range(lc, lc + ld + 1) for c in comps(l)]
+
+    def emit(self):
+        lab, lcd = self.la + self.lb, self.lc + self.ld
+        lines = []
+        names = {}
+        for d, ax in enumerate(XYZ):
+            tab = {}
+            for k in range(lcd + 1):
+                for i in range(lab + 1):
+                    if i == 0 and k == 0:
+                        tab[(i, k)] = "pref" if d == 0 else "(R)1"
+                        continue
+                    if k == 0:
+                        terms = ["XA%s*%s" % (ax, tab[(i - 1, 0)])]
+                        if i - 1 > 0:
+                            terms.append("(R)%d*s11*%s" % (i - 1, tab[(i - 2, 0)]))
+                    else:
+                        terms = ["XC%s*%s" % (ax, tab[(i, k - 1)])]
+                        if i > 0:
+                            terms.append("(R)%d*s12*%s" % (i, tab[(i - 1, k - 1)]))
+                        if k - 1 > 0:
+                            terms.append("(R)%d*s22*%s" % (k - 1, tab[(i, k - 2)]))
+                    name = "g%s%d_%d" % (ax, i, k)
+                    lines.append("  const R %s = %s;" % (name, " + ".join(terms)))
+                    tab[(i, k)] = name
+            names[ax] = tab
+        nf = len(self.fs)
+        for ie, e in enumerate(self.es):
+            for jf, f in enumerate(self.fs):
+                fac = [names[ax][(e[d], f[d])] for d, ax in enumerate(XYZ)]
+                fac = [x for x in fac if x != "(R)1"]
+                lines.append("  acc[%d] += %s;" % (ie * nf + jf, "*".join(fac)))
+        sig = ("template <typename R>\n__device__ __forceinline__ void g12vrr_%d%d%d%d(R XAx, R XAy, R XAz, "
+               "R XCx, R XCy, R XCz, R s11, R s12, R s22, R pref, R* __restrict__ acc) {"
+               % (self.la, self.lb, self.lc, self.ld))
+        unused = "  (void)XAx; (void)XAy; (void)XAz; (void)XCx; (void)XCy; (void)XCz; (void)s11; (void)s12; (void)s22;"
+        return "\n".join([sig, unused] + lines + ["}"])
+
+
 class HRRGen:
     def __init__(self, la, lb, lc, ld):
         self.la, self.lb, self.lc, self.ld = la, lb, lc, ld
@@ -240,6 +300,21 @@ void launch_cst_%(k)s(const CouplingStoreArgs& args, hipStream_t s) { launch_cst
 """
 
 
+G12_TU = """// Generated by gen_eri.py -- do not edit.
+// Kernels of quartet class (%(a)d%(b)d|%(c)d%(d)d): G12 (Gaussian-geminal) Fock build and coupling.
+#include "../eri_g12.h"
+
+namespace lowdin {
+#include "eri_%(k)s.inc"
+LOWDIN_ERI_CLASS(%(a)d, %(b)d, %(c)d, %(d)d)
+LOWDIN_G12_CLASS(%(a)d, %(b)d, %(c)d, %(d)d)
+
+void launch_gjk_%(k)s(const G12Args& args, hipStream_t s) { launch_g12_t<%(a)d, %(b)d, %(c)d, %(d)d>(args, s); }
+void launch_gcp_%(k)s(const G12CouplingArgs& args, hipStream_t s) { launch_g12cp_t<%(a)d, %(b)d, %(c)d, %(d)d>(args, s); }
+}  // namespace lowdin
+"""
+
+
 def main(outdir):
     pcs, classes = canonical_classes()
     gdir = os.path.join(outdir, "gen")
@@ -255,7 +330,7 @@ def main(outdir):
         inc = ["// Generated by gen_eri.py -- do not edit.",
                "// class (%d%d|%d%d): L = %d, %d accumulators, VRR %d flops/primitive quartet, HRR %d flops"
                % (la, lb, lc, ld, L, nacc, v.flops, h.flops),
-               "#define NACC_%s %d" % (k, nacc), code, hcode, ""]
+               "#define NACC_%s %d" % (k, nacc), code, hcode, G12Gen(la, lb, lc, ld).emit(), ""]
         with open(os.path.join(gdir, "eri_%s.inc" % k), "w") as fh:
             fh.write("\n".join(inc))
         diag = (la, lb) == (lc, ld)
@@ -266,6 +341,8 @@ def main(outdir):
         with open(os.path.join(gdir, "k_%s.hip" % k), "w") as fh:
             fh.write(KERNEL_TU % {"a": la, "b": lb, "c": lc, "d": ld, "k": k,
                                   "schw": ", Schwarz" if diag else "", "schwfn": schwfn})
+        with open(os.path.join(gdir, "g_%s.hip" % k), "w") as fh:
+            fh.write(G12_TU % {"a": la, "b": lb, "c": lc, "d": ld, "k": k})
         with open(os.path.join(gdir, "s_%s.hip" % k), "w") as fh:
             fh.write(STORE_TU % {"a": la, "b": lb, "c": lc, "d": ld, "k": k})
         table[k] = {"L": L, "nacc": nacc, "vrr_flops": v.flops, "hrr_flops": h.flops}
@@ -283,6 +360,8 @@ def main(outdir):
         lines.append("void launch_cp_%s(const CouplingArgs&, hipStream_t);" % k)
         lines.append("void launch_st_%s(const StoreArgs&, hipStream_t);" % k)
         lines.append("void launch_cst_%s(const CouplingStoreArgs&, hipStream_t);" % k)
+        lines.append("void launch_gjk_%s(const G12Args&, hipStream_t);" % k)
+        lines.append("void launch_gcp_%s(const G12CouplingArgs&, hipStream_t);" % k)
     for (la, lb) in pcs:
         lines.append("void launch_schwarz_%d%d(PairInfo*, int, const double*, const double*, const double*, hipStream_t);"
                      % (la, lb))
@@ -310,6 +389,16 @@ def main(outdir):
     lines.append("  switch (la * 1000 + lb * 100 + lc * 10 + ld) {")
     for (la, lb, lc, ld) in classes:
         lines.append("    case %d: launch_cst_%d%d%d%d(a, s); return true;" % (la * 1000 + lb * 100 + lc * 10 + ld, la, lb, lc, ld))
+    lines.append("    default: return false;\n  }\n}")
+    lines.append("inline bool dispatch_g12(int la, int lb, int lc, int ld, const G12Args& a, hipStream_t s) {")
+    lines.append("  switch (la * 1000 + lb * 100 + lc * 10 + ld) {")
+    for (la, lb, lc, ld) in classes:
+        lines.append("    case %d: launch_gjk_%d%d%d%d(a, s); return true;" % (la * 1000 + lb * 100 + lc * 10 + ld, la, lb, lc, ld))
+    lines.append("    default: return false;\n  }\n}")
+    lines.append("inline bool dispatch_g12cp(int la, int lb, int lc, int ld, const G12CouplingArgs& a, hipStream_t s) {")
+    lines.append("  switch (la * 1000 + lb * 100 + lc * 10 + ld) {")
+    for (la, lb, lc, ld) in classes:
+        lines.append("    case %d: launch_gcp_%d%d%d%d(a, s); return true;" % (la * 1000 + lb * 100 + lc * 10 + ld, la, lb, lc, ld))
     lines.append("    default: return false;\n  }\n}")
     lines.append("inline bool dispatch_schwarz(int la, int lb, PairInfo* p, int np, const double* prim, const double* boys, "
                  "const double* norma, hipStream_t s) {")

@@ -210,7 +210,7 @@ struct Species {
   DevBuf<double> d_D, d_G, d_Dother, d_R;
   TaskSet store_ts;                  // wavefront tasks of the integral store / transform
   uint64_t store_gen = ~0ull;        // sig_gen the store tasks were built from
-  DevBuf<double> d_Dn1, d_cvec, d_store;
+  DevBuf<double> d_Dn1, d_cvec, d_store, d_Graw1;
   lowdin_fock_stats stats{};
   int mixed = 0;            // mixed precision: FP32 quartets below tau
   double tau = 0.0;
@@ -1091,6 +1091,135 @@ static void enqueue_coupling_store(Species& a, Species& b, const double* dC, dou
   HIPCHK(hipGetLastError());
 }
 
+// ------------------------------------------ Gaussian-geminal (G12) builds
+static G12Params make_gem(const double* coefficients, const double* exponents, int pot_size) {
+  if (pot_size < 0 || pot_size > kMaxGeminal)
+    throw std::runtime_error("pot_size must be in 0.." + std::to_string(kMaxGeminal));
+  if (pot_size > 0 && (!coefficients || !exponents)) throw std::runtime_error("null geminal parameters");
+  G12Params g{};
+  g.n = pot_size;
+  for (int k = 0; k < pot_size; ++k) {
+    g.c[k] = coefficients[k];
+    g.g[k] = exponents[k];
+    if (!(exponents[k] >= 0.0)) throw std::runtime_error("geminal exponents must be >= 0");
+  }
+  return g;
+}
+
+// dG (n x n) = 0.25 (Graw + Graw^T), Graw the six J/K updates of every
+// quartet of the pair list with the geminal operator (compute_g12_direct,
+// Libint2Iface.cpp:2193-2365: no density screen), enqueued on st.
+static void enqueue_g12(Species& s, const double* dD, double* dG, double factor, const G12Params& gem,
+                        hipStream_t st, lowdin_fock_stats* stt) {
+  Context& c = ctx();
+  ensure_sig(s, st);
+  TaskSet& ts = store_taskset(s, st);
+  const int n = s.nbasis;
+  double* graw = s.d_Graw1.ensure(std::max<size_t>((size_t)n * n, 1));
+  launch_fill_zero(graw, (size_t)n * n, st);
+  hipStream_t hs = c.side[0];
+  HIPCHK(hipEventRecord(c.fork, st));
+  HIPCHK(hipStreamWaitEvent(hs, c.fork, 0));
+  for (const TaskSet::Launch& L : ts.launches) {
+    G12Args g{};
+    g.tasks = ts.d_tasks.p + L.off;
+    g.ntasks = (int)L.count;
+    g.n = n;
+    g.bra = s.sig.d_pairs.p + s.sig.off[L.cbra];
+    g.ket = s.sig.d_pairs.p + s.sig.off[L.cket];
+    g.prim = s.sig.d_prims.p;
+    g.norma = s.d_norma.p;
+    g.D = dD;
+    g.Graw = graw;
+    g.same_class = L.cbra == L.cket ? 1 : 0;
+    g.half_factor = 0.5 * factor;
+    g.gem = gem;
+    if (!dispatch_g12(L.la, L.lb, L.lc, L.ld, g, hs)) throw std::runtime_error("no G12 kernel for class");
+    count_tasks(ts, L, s.sig.pairs.data() + s.sig.off[L.cbra], s.sig.pairs.data() + s.sig.off[L.cket], stt);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(c.join[0], hs));
+  HIPCHK(hipStreamWaitEvent(st, c.join[0], 0));
+  launch_symmetrize(graw, n, 1.0, dG, st);
+  HIPCHK(hipGetLastError());
+}
+
+// dRa (n_a x n_a) = G12 coupling on species a from b's density dDb
+// (compute_g12inter_direct, Libint2Iface.cpp:2367-2550; the layout and
+// element sets of the Coulomb coupling), zeroed here, enqueued on st.
+static void enqueue_g12_coupling(Species& a, Species& b, const double* dDb, double* dRa, const G12Params& gem,
+                                 hipStream_t st, lowdin_fock_stats* stt) {
+  Context& c = ctx();
+  ensure_all(a, st);
+  ensure_all(b, st);
+  TaskSet* ts = coupling_taskset(a, b, st);
+  launch_fill_zero(dRa, (size_t)a.nbasis * a.nbasis, st);
+  hipStream_t hs = c.side[0];
+  HIPCHK(hipEventRecord(c.fork, st));
+  HIPCHK(hipStreamWaitEvent(hs, c.fork, 0));
+  for (const TaskSet::Launch& L : ts->launches) {
+    const bool a_bra = L.bra_is_a != 0;
+    Species& sb = a_bra ? a : b;
+    Species& sk = a_bra ? b : a;
+    G12CouplingArgs ga{};
+    CouplingArgs& g = ga.cp;
+    g.tasks = ts->d_tasks.p + L.off;
+    g.ntasks = (int)L.count;
+    g.bra = sb.all.d_pairs.p + sb.all.off[L.cbra];
+    g.ket = sk.all.d_pairs.p + sk.all.off[L.cket];
+    g.bprim = sb.all.d_prims.p;
+    g.kprim = sk.all.d_prims.p;
+    g.nbra_norma = sb.d_norma.p;
+    g.nket_norma = sk.d_norma.p;
+    g.nbra = sb.nbasis;
+    g.nket = sk.nbasis;
+    g.boys = c.boys.p;
+    g.drop = 1.0e-10;  // Libint2Iface.cpp:2496-2497
+    if (a_bra) {
+      g.rbra = dRa;
+      g.dket = dDb;
+    } else {
+      g.rket = dRa;
+      g.dbra = dDb;
+    }
+    ga.gem = gem;
+    if (!dispatch_g12cp(L.la, L.lb, L.lc, L.ld, ga, hs)) throw std::runtime_error("no G12 coupling kernel for class");
+    count_tasks(*ts, L, sb.all.pairs.data() + sb.all.off[L.cbra], sk.all.pairs.data() + sk.all.off[L.cket], stt);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(c.join[0], hs));
+  HIPCHK(hipStreamWaitEvent(st, c.join[0], 0));
+}
+
+// Host-pointer G12 build (b null: intra-species on a with a's density;
+// otherwise the coupling on a from b's density).
+static void run_g12(Species& a, Species* b, const double* dens, double* result, double factor,
+                    const G12Params& gem) {
+  Context& c = ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  const double t0 = now_ms();
+  ensure_context();
+  hipStream_t st = c.stream;
+  const size_t na2 = (size_t)a.nbasis * a.nbasis;
+  const size_t nd2 = b ? (size_t)b->nbasis * b->nbasis : na2;
+  a.d_D.upload(dens, nd2, st);
+  double* dR = a.d_G.ensure(std::max<size_t>(na2, 1));
+  lowdin_fock_stats stt{};
+  HIPCHK(hipEventRecord(c.ev0, st));
+  if (b)
+    enqueue_g12_coupling(a, *b, a.d_D.p, dR, gem, st, &stt);
+  else
+    enqueue_g12(a, a.d_D.p, dR, factor, gem, st, &stt);
+  HIPCHK(hipEventRecord(c.ev1, st));
+  HIPCHK(hipMemcpyAsync(result, dR, na2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  float kms = 0.f;
+  HIPCHK(hipEventElapsedTime(&kms, c.ev0, c.ev1));
+  stt.kernel_ms = kms;
+  stt.total_ms = now_ms() - t0;
+  a.stats = stt;
+}
+
 // Row p of an n x n column-major coefficient matrix, C(p, u) = coeff[p + u n]
 // (the Eigen copy of Libint2Iface.cpp:3535-3537, read as C(p, bf)).
 static std::vector<double> coeff_row(const double* coeff, int n, int p) {
@@ -1323,6 +1452,71 @@ void LibintInterface_compute_coupling_directIT(void* lint, void* olint, double* 
     lowdin::set_last_error(std::string("compute_coupling_directIT: ") + e.what());
     const size_t n = s->nbasis, on = o->nbasis;
     lowdin::fill_nan(results, n * on * on);
+  }
+}
+
+// Libint2Iface.cpp:3687-3706: result = J[D] + factor K[D] with the
+// contracted Gaussian geminal sum_k coefficients[k] exp(-exponents[k] r12^2).
+void libintinterface_compute_g12_direct(void* lint, double* dens, double* result, double* factor,
+                                        const double* coefficients, const double* exponents, const int pot_size) {
+  Species* s = static_cast<Species*>(lint);
+  try {
+    const lowdin::G12Params gem = lowdin::make_gem(coefficients, exponents, pot_size);
+    lowdin::run_g12(*s, nullptr, dens, result, *factor, gem);
+    lowdin::g_last_error.clear();
+  } catch (const std::exception& e) {
+    lowdin::set_last_error(std::string("compute_g12_direct: ") + e.what());
+    lowdin::fill_nan(result, (size_t)s->nbasis * s->nbasis);
+  }
+}
+
+// Libint2Iface.cpp:3708-3734: the G12 coupling on lint's species from the
+// density of olint's species, in the layout of compute_coupling_direct.
+void libintinterface_compute_g12inter_direct(void* lint, void* olint, double* dens, double* result,
+                                             const double* coefficients, const double* exponents,
+                                             const int pot_size) {
+  Species* s = static_cast<Species*>(lint);
+  try {
+    const lowdin::G12Params gem = lowdin::make_gem(coefficients, exponents, pot_size);
+    lowdin::run_g12(*s, static_cast<Species*>(olint), dens, result, 0.0, gem);
+    lowdin::g_last_error.clear();
+  } catch (const std::exception& e) {
+    lowdin::set_last_error(std::string("compute_g12inter_direct: ") + e.what());
+    lowdin::fill_nan(result, (size_t)s->nbasis * s->nbasis);
+  }
+}
+
+// Device-resident G12 build: olint NULL = intra-species (d_dens n x n of
+// lint, factor), else the coupling on lint from olint's density d_dens.
+int lowdin_fock_g12_device(void* lint, void* olint, const double* d_dens, double* d_result, double factor,
+                           const double* coefficients, const double* exponents, int pot_size) {
+  Species* s = static_cast<Species*>(lint);
+  Species* o = static_cast<Species*>(olint);
+  lowdin::Context& c = lowdin::ctx();
+  try {
+    const lowdin::G12Params gem = lowdin::make_gem(coefficients, exponents, pot_size);
+    std::lock_guard<std::mutex> lk(c.mu);
+    const double t0 = lowdin::now_ms();
+    lowdin::ensure_context();
+    hipStream_t st = c.stream;
+    lowdin_fock_stats stt{};
+    HIPCHK(hipEventRecord(c.ev0, st));
+    if (o)
+      lowdin::enqueue_g12_coupling(*s, *o, d_dens, d_result, gem, st, &stt);
+    else
+      lowdin::enqueue_g12(*s, d_dens, d_result, factor, gem, st, &stt);
+    HIPCHK(hipEventRecord(c.ev1, st));
+    HIPCHK(hipStreamSynchronize(st));
+    float kms = 0.f;
+    HIPCHK(hipEventElapsedTime(&kms, c.ev0, c.ev1));
+    stt.kernel_ms = kms;
+    stt.total_ms = lowdin::now_ms() - t0;
+    s->stats = stt;
+    lowdin::g_last_error.clear();
+    return 0;
+  } catch (const std::exception& e) {
+    lowdin::set_last_error(std::string("g12_device: ") + e.what());
+    return -1;
   }
 }
 

@@ -2,9 +2,9 @@
 //
 // openLOWDIN's Fortran interface block (src/ints/Libint2Interface.f90:165-373)
 // binds every entry point of src/ints/Libint2Iface.h:350-453.  The direct
-// Fock path and its integral-store / first-quarter-transform variants are
-// implemented in lowdin_fock.cpp; the disk, alpha-beta and G12 variants
-// below are outside this library's scope.  They exist so that a Fortran build of openLOWDIN
+// Fock path, its integral-store / first-quarter-transform variants and the
+// direct G12 builds are implemented in lowdin_fock.cpp; the disk,
+// alpha-beta and G12 store variants below are outside this library's scope.  They exist so that a Fortran build of openLOWDIN
 // links against liblowdin_fock.so unchanged; calling one records an error
 // (lowdin_fock_last_error) and prints it, leaving outputs untouched.
 #include <cstdio>
@@ -32,13 +32,6 @@ void libintinterface_compute_g12_disk(void*, const char*, const double*, const d
 }
 void libintinterface_compute_g12inter_disk(void*, void*, const char*, const double*, const double*, const int) {
   LOWDIN_UNSUPPORTED("libintinterface_compute_g12inter_disk");
-}
-void libintinterface_compute_g12_direct(void*, double*, double*, double*, const double*, const double*, const int) {
-  LOWDIN_UNSUPPORTED("libintinterface_compute_g12_direct");
-}
-void libintinterface_compute_g12inter_direct(void*, void*, double*, double*, const double*, const double*,
-                                             const int) {
-  LOWDIN_UNSUPPORTED("libintinterface_compute_g12inter_direct");
 }
 void libintinterface_compute_g12_directAll(void*, double*, double*, const double*, const double*, const int) {
   LOWDIN_UNSUPPORTED("libintinterface_compute_g12_directAll");

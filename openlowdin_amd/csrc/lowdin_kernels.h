@@ -122,6 +122,42 @@ struct CouplingStoreArgs {
   double drop;             // dropped if |raw| < drop
 };
 
+// Contracted Gaussian-type geminal sum_k c[k] exp(-g[k] r12^2).
+constexpr int kMaxGeminal = 64;
+struct G12Params {
+  int n;
+  int pad;
+  double c[kMaxGeminal];
+  double g[kMaxGeminal];
+};
+
+// Intra-species G12 J/K build (compute_g12_direct): the Fock build's
+// quartet set without the density screen; Graw (n x n) accumulates the six
+// updates with degeneracy and 0.5 * factor exchange weight, symmetrised
+// afterwards as 0.25 (Graw + Graw^T).
+struct G12Args {
+  const Task* tasks;
+  int ntasks;
+  int n;
+  const PairInfo* bra;
+  const PairInfo* ket;
+  const double* prim;
+  const double* norma;
+  const double* D;
+  double* Graw;
+  int same_class;
+  int pad;
+  double half_factor;
+  G12Params gem;
+};
+
+// Inter-species G12 coupling (compute_g12inter_direct): the coupling
+// build's quartets, element sets and drop with the geminal operator.
+struct G12CouplingArgs {
+  CouplingArgs cp;
+  G12Params gem;
+};
+
 // Raw overlap block <A|B> (row-major, A's components slowest) of two shells
 // with l <= 2, from primitives whose coefficients already carry N(alpha,l).
 // Obara-Saika 1-D recursion S(i+1,j) = PA S(i,j) + (i S(i-1,j) + j S(i,j-1))/(2p).

@@ -73,6 +73,10 @@ def lib():
         L.LibintInterface_compute_coupling_directIT.argtypes = [vp, vp, dp, dp, dp, ip]
         L.lowdin_fock_store_device.argtypes = [vp, vp, C.c_int, vp, vp, vp]
         L.lowdin_fock_store_device.restype = C.c_int
+        L.libintinterface_compute_g12_direct.argtypes = [vp, dp, dp, dp, dp, dp, C.c_int]
+        L.libintinterface_compute_g12inter_direct.argtypes = [vp, vp, dp, dp, dp, dp, C.c_int]
+        L.lowdin_fock_g12_device.argtypes = [vp, vp, vp, vp, C.c_double, dp, dp, C.c_int]
+        L.lowdin_fock_g12_device.restype = C.c_int
         L.lowdin_fock_last_error.restype = C.c_char_p
         L.lowdin_fock_get_stats.argtypes = [vp, C.POINTER(Stats)]
         L.lowdin_fock_shellpair_list.argtypes = [vp, ip, ip, C.c_int]
@@ -163,6 +167,14 @@ def _fortran(m: np.ndarray, n: int) -> np.ndarray:
     if a.shape != (n, n):
         raise ValueError("expected an %d x %d matrix, got %s" % (n, n, a.shape))
     return a
+
+
+def _geminal(coefficients, exponents):
+    c = np.ascontiguousarray(coefficients, dtype=np.float64).reshape(-1)
+    g = np.ascontiguousarray(exponents, dtype=np.float64).reshape(-1)
+    if c.shape != g.shape:
+        raise ValueError("geminal coefficients and exponents differ in length")
+    return (c if len(c) else np.zeros(1)), (g if len(g) else np.zeros(1))
 
 
 class Libint2Interface:
@@ -281,6 +293,52 @@ class Libint2Interface:
                                                         C.byref(pp))
         _check("compute_coupling_directIT")
         return GG.reshape(n, no, no)
+
+    # Libint2Interface_computeG12Intraspecies_direct
+    # (Libint2Interface.f90:935-992 -> Libint2Iface.cpp:3687-3706)
+    def compute_g12_intraspecies_direct(self, density: np.ndarray, factor: float, coefficients,
+                                        exponents) -> np.ndarray:
+        """J + factor K with the contracted Gaussian geminal
+        sum_k coefficients[k] exp(-exponents[k] r12^2) (the interparticle
+        potential of the species with itself); the pair list is rebuilt
+        first, as the Fortran wrapper does."""
+        n = self.nbasis
+        D = _fortran(density, n)
+        c, g = _geminal(coefficients, exponents)
+        G = np.zeros((n, n), dtype=np.float64, order="F")
+        f = C.c_double(factor)
+        lib().LibintInterface_init_2body_ints(self.this)
+        _check("init_2body_ints")
+        lib().libintinterface_compute_g12_direct(self.this, _dptr(D), _dptr(G), C.byref(f), _dptr(c), _dptr(g),
+                                                 len(coefficients))
+        _check("compute_g12_direct")
+        return G
+
+    # Libint2Interface_computeG12Interspecies_direct
+    # (Libint2Interface.f90:995-1061 -> Libint2Iface.cpp:3708-3734)
+    def compute_g12_interspecies_direct(self, other: "Libint2Interface", density_other: np.ndarray, coefficients,
+                                        exponents) -> np.ndarray:
+        """The geminal coupling on this species from the other's density, in
+        the layout of compute_2body_interspecies_direct."""
+        n = self.nbasis
+        D = _fortran(density_other, other.nbasis)
+        c, g = _geminal(coefficients, exponents)
+        B = np.zeros((n, n), dtype=np.float64, order="F")
+        lib().libintinterface_compute_g12inter_direct(self.this, other.this, _dptr(D), _dptr(B), _dptr(c), _dptr(g),
+                                                      len(coefficients))
+        _check("compute_g12inter_direct")
+        return B
+
+    def g12_device(self, d_dens: int, d_result: int, coefficients, exponents, factor: float = 0.0,
+                   other: "Libint2Interface" = None):
+        """Device-resident G12 build (other None: intra-species with factor;
+        otherwise the coupling from the other species' density)."""
+        c, g = _geminal(coefficients, exponents)
+        rc = lib().lowdin_fock_g12_device(self.this, other.this if other is not None else None, d_dens, d_result,
+                                          float(factor), _dptr(c), _dptr(g), len(coefficients))
+        if rc != 0:
+            _check("g12_device")
+            raise LowdinFockError("g12_device failed")
 
     def store_device(self, which: int, d_out: int, d_dens: int = 0, d_crow: int = 0,
                      other: "Libint2Interface" = None):

@@ -29,6 +29,8 @@
  *     permuted/non-permuted scatter, no symmetrisation):
  *                                         src/ints/Libint2Iface.cpp:1111-1294
  *   - 1-body S/T/V with norma:            src/ints/Libint2Iface.cpp:163-217
+ *   - G12 (Gaussian-geminal) Fock and coupling builds, the same loops with
+ *     sum_k c_k exp(-g_k r12^2) for 1/r12: src/ints/Libint2Iface.cpp:2193-2550
  *   - integral store over the Fock build's quartets (|raw| > 1e-10, eight
  *     images, norma):                     src/ints/Libint2Iface.cpp:2552-2742
  *   - inter-species integral store (all pairs, bf1 <= bf2 / obf1 <= obf2,
@@ -63,6 +65,13 @@ typedef struct {
   const double *coefs; /* raw (file) contraction coefficients */
   const double *xyz;   /* [3*nshell] shell origins, bohr */
 } orc_basis;
+
+/* Contracted Gaussian-type geminal sum_k c[k] exp(-g[k] r12^2) (NULL: 1/r12). */
+typedef struct {
+  int n;
+  const double *c;
+  const double *g;
+} orc_gem;
 
 static inline int ncart(int l) { return (l + 1) * (l + 2) / 2; }
 
@@ -179,12 +188,58 @@ static void hermite_r(int L, double alpha, const double pq[3], double *Rout /* R
       for (int v = 0; v <= L - t - u; ++v) Rout[IX(t, u, v)] = prev[IX(t, u, v)];
 }
 
+/* The same table for the geminal operator: R[t][u][v] = d^t/dX^t d^u/dY^u
+ * d^v/dZ^v of sum_k c_k (pi^2/D_k)^(3/2) exp(-rho_k |PQ|^2), D_k = pq +
+ * g_k (p + q), rho_k = pq g_k / D_k -- the integral of the operator between
+ * two unit s-type Hermite Gaussians at P and Q, differentiated.  Each axis
+ * factor h_t = d^t/dX^t exp(-rho X^2) obeys h_{t+1} = -2 rho (X h_t + t h_{t-1}). */
+static void hermite_r_gem(int L, double p, double q, const double pq[3], const orc_gem *gem, double *Rout) {
+  for (int t = 0; t <= L; ++t)
+    for (int u = 0; u <= L - t; ++u)
+      for (int v = 0; v <= L - t - u; ++v) Rout[IX(t, u, v)] = 0.0;
+  for (int k = 0; k < gem->n; ++k) {
+    const double g = gem->g[k], D = p * q + g * (p + q), rho = p * q * g / D;
+    const double w = gem->c[k] * pow(ORC_PI * ORC_PI / D, 1.5);
+    double h[3][ORC_MAXLT + 2];
+    for (int d = 0; d < 3; ++d) {
+      const double X = pq[d];
+      h[d][0] = exp(-rho * X * X);
+      if (L >= 1) h[d][1] = -2.0 * rho * X * h[d][0];
+      for (int t = 1; t < L; ++t) h[d][t + 1] = -2.0 * rho * (X * h[d][t] + t * h[d][t - 1]);
+    }
+    for (int t = 0; t <= L; ++t)
+      for (int u = 0; u <= L - t; ++u)
+        for (int v = 0; v <= L - t - u; ++v) Rout[IX(t, u, v)] += w * h[0][t] * h[1][u] * h[2][v];
+  }
+}
+
 /* ------------------------------------------------ primitive ERIs */
 /* Unnormalised primitive Cartesian ERIs [ab|cd] for x^l exp(-alpha r^2)
  * primitives, all components, index ((ia*nb + ib)*nc + ic)*nd + id. */
+static void prim_two_body(int la, int lb, int lc, int ld, const double *A, const double *B, const double *C,
+                          const double *D, double a, double b, double c, double d, const orc_gem *gem,
+                          double *out);
+
 void orc_prim_eri(int la, int lb, int lc, int ld, const double *A, const double *B,
                   const double *C, const double *D, double a, double b, double c,
                   double d, double *out) {
+  prim_two_body(la, lb, lc, ld, A, B, C, D, a, b, c, d, NULL, out);
+}
+
+/* Primitive integrals of the geminal operator, same layout as orc_prim_eri. */
+void orc_prim_g12(int la, int lb, int lc, int ld, const double *A, const double *B, const double *C,
+                  const double *D, double a, double b, double c, double d, int ng, const double *gc,
+                  const double *gg, double *out) {
+  const orc_gem gem = {ng, gc, gg};
+  prim_two_body(la, lb, lc, ld, A, B, C, D, a, b, c, d, &gem, out);
+}
+
+/* McMurchie-Davidson: (ab|op|cd) = sum E^ab_tuv E^cd_tau,nu,phi (-1)^(tau+nu+phi)
+ * R_{t+tau,u+nu,v+phi}, R the operator's Hermite table (Coulomb: Boys-based
+ * with prefactor 2 pi^(5/2) / (pq sqrt(p+q)); geminal: hermite_r_gem). */
+static void prim_two_body(int la, int lb, int lc, int ld, const double *A, const double *B, const double *C,
+                          const double *D, double a, double b, double c, double d, const orc_gem *gem,
+                          double *out) {
   const double p = a + b, q = c + d, rho = p * q / (p + q);
   double P[3], Q[3], PQ[3];
   double ab2 = 0, cd2 = 0;
@@ -195,7 +250,7 @@ void orc_prim_eri(int la, int lb, int lc, int ld, const double *A, const double 
     ab2 += (A[k] - B[k]) * (A[k] - B[k]);
     cd2 += (C[k] - D[k]) * (C[k] - D[k]);
   }
-  const double pref = 2.0 * pow(ORC_PI, 2.5) / (p * q * sqrt(p + q)) *
+  const double pref = (gem ? 1.0 : 2.0 * pow(ORC_PI, 2.5) / (p * q * sqrt(p + q))) *
                       exp(-a * b / p * ab2) * exp(-c * d / q * cd2);
   static etab Eab[3], Ecd[3];
 #pragma omp threadprivate(Eab, Ecd)
@@ -206,7 +261,10 @@ void orc_prim_eri(int la, int lb, int lc, int ld, const double *A, const double 
   const int L = la + lb + lc + ld;
   static double R[RD * RD * RD];
 #pragma omp threadprivate(R)
-  hermite_r(L, rho, PQ, R);
+  if (gem)
+    hermite_r_gem(L, p, q, PQ, gem, R);
+  else
+    hermite_r(L, rho, PQ, R);
   int ca[ORC_MAXNC][3], cb[ORC_MAXNC][3], cc[ORC_MAXNC][3], cdd[ORC_MAXNC][3];
   cart_comps(la, ca); cart_comps(lb, cb); cart_comps(lc, cc); cart_comps(ld, cdd);
   const int na = ncart(la), nb = ncart(lb), nc = ncart(lc), nd = ncart(ld);
@@ -263,8 +321,23 @@ static void shell_xyz(const orc_basis *bs, int s, double X[3]) {
  * With prim_eps > 0, primitive pairs whose Gaussian-product factor
  * exp(-ab/(a+b) |AB|^2) is below prim_eps are skipped (the same pair-level
  * truncation the GPU build applies at 1e-30); prim_eps = 0 keeps all. */
+static void shell_block(const orc_basis *b12, int s1, int s2, const orc_basis *b34, int s3, int s4,
+                        double prim_eps, const orc_gem *gem, double *out);
+
 void orc_eri_shell(const orc_basis *b12, int s1, int s2, const orc_basis *b34, int s3,
                    int s4, double prim_eps, double *out) {
+  shell_block(b12, s1, s2, b34, s3, s4, prim_eps, NULL, out);
+}
+
+/* Raw contracted integrals of the geminal operator for (s1 s2 | s3 s4). */
+void orc_g12_shell(const orc_basis *b12, int s1, int s2, const orc_basis *b34, int s3, int s4, int ng,
+                   const double *gc, const double *gg, double *out) {
+  const orc_gem gem = {ng, gc, gg};
+  shell_block(b12, s1, s2, b34, s3, s4, 0.0, &gem, out);
+}
+
+static void shell_block(const orc_basis *b12, int s1, int s2, const orc_basis *b34, int s3, int s4,
+                        double prim_eps, const orc_gem *gem, double *out) {
   const int l1 = b12->l[s1], l2 = b12->l[s2], l3 = b34->l[s3], l4 = b34->l[s4];
   const int n = ncart(l1) * ncart(l2) * ncart(l3) * ncart(l4);
   double A[3], B[3], C[3], D[3];
@@ -291,7 +364,7 @@ void orc_eri_shell(const orc_basis *b12, int s1, int s2, const orc_basis *b34, i
           const double d = b34->exps[b34->poff[s4] + l];
           if (c * d / (c + d) * cd2 > cut) continue;
           const double cd = b34->coefs[b34->poff[s4] + l] * orc_prim_norm(d, l4);
-          orc_prim_eri(l1, l2, l3, l4, A, B, C, D, a, b, c, d, prim);
+          prim_two_body(l1, l2, l3, l4, A, B, C, D, a, b, c, d, gem, prim);
           const double w = ca * cb * cc * cd;
           for (int m = 0; m < n; ++m) out[m] += w * prim[m];
         }
@@ -409,10 +482,34 @@ void orc_schwarz(const orc_basis *bs, double *Q) {
  * counters (optional, 2 longs): [0] computed shell quartets, [1] primitive
  * quartets before primitive screening.  s1_stride > 1 restricts the outer
  * shell loop to s1 = 0 mod s1_stride (a bounded sample for CPU timing). */
+static long direct_build(const orc_basis *bs, const int *pairs_s1, const int *pairs_s2, int npairs,
+                         const double *Q, const double *Dmat, double factor, int screen, double precision,
+                         double prim_eps, int nthreads, int s1_stride, const orc_gem *gem, double *G,
+                         long *counters);
+
 long orc_compute_2body_direct(const orc_basis *bs, const int *pairs_s1, const int *pairs_s2,
                               int npairs, const double *Q, const double *Dmat, double factor,
                               int screen, double precision, double prim_eps, int nthreads,
                               int s1_stride, double *G, long *counters) {
+  return direct_build(bs, pairs_s1, pairs_s2, npairs, Q, Dmat, factor, screen, precision, prim_eps, nthreads,
+                      s1_stride, NULL, G, counters);
+}
+
+/* Reference LibintInterface::compute_g12_direct (Libint2Iface.cpp:2193-2365):
+ * the loops, degeneracy, updates and symmetrisation of the direct build with
+ * the geminal operator and no density screen. */
+long orc_compute_g12_direct(const orc_basis *bs, const int *pairs_s1, const int *pairs_s2, int npairs,
+                            const double *Dmat, double factor, int ng, const double *gc, const double *gg,
+                            int nthreads, double *G) {
+  const orc_gem gem = {ng, gc, gg};
+  return direct_build(bs, pairs_s1, pairs_s2, npairs, NULL, Dmat, factor, 0, 0.0, 0.0, nthreads, 1, &gem, G,
+                      NULL);
+}
+
+static long direct_build(const orc_basis *bs, const int *pairs_s1, const int *pairs_s2, int npairs,
+                         const double *Q, const double *Dmat, double factor, int screen, double precision,
+                         double prim_eps, int nthreads, int s1_stride, const orc_gem *gem, double *G,
+                         long *counters) {
   const int ns = bs->nshell, n = orc_nbasis(bs);
   int *off = (int *)malloc(sizeof(int) * ns);
   shell_offsets(bs, off);
@@ -486,7 +583,7 @@ long orc_compute_2body_direct(const orc_basis *bs, const int *pairs_s1, const in
             nprimq += (long)bs->nprim[s1] * bs->nprim[s2] * bs->nprim[s3] * bs->nprim[s4];
             const double deg = (s1 == s2 ? 1.0 : 2.0) * (s3 == s4 ? 1.0 : 2.0) *
                                ((s1 == s3) ? (s2 == s4 ? 1.0 : 2.0) : 2.0);
-            orc_eri_shell(bs, s1, s2, bs, s3, s4, prim_eps, buf);
+            shell_block(bs, s1, s2, bs, s3, s4, prim_eps, gem, buf);
             for (int f1 = 0, f1234 = 0; f1 < n1; ++f1) {
               const int b1 = bf1f + f1;
               for (int f2 = 0; f2 < n2; ++f2) {
@@ -531,9 +628,27 @@ long orc_compute_2body_direct(const orc_basis *bs, const int *pairs_s1, const in
  * lower triangle (row >= col) filled, exactly as the reference returns it.
  * counts (optional): number of raw ERIs kept after the 1e-10 drop, and the
  * number of ERIs inside 1e-3 relative of the cutoff. */
+static void coupling_build(const orc_basis *outer, const orc_basis *inner, const double *Dmat, int permuted,
+                           double drop, int nthreads, const orc_gem *gem, double *Bout, long *counts);
+
 void orc_compute_coupling_direct(const orc_basis *outer, const orc_basis *inner,
                                  const double *Dmat, int permuted, double drop,
                                  int nthreads, double *Bout, long *counts) {
+  coupling_build(outer, inner, Dmat, permuted, drop, nthreads, NULL, Bout, counts);
+}
+
+/* Reference LibintInterface::compute_g12inter_direct
+ * (Libint2Iface.cpp:2367-2550): the coupling build with the geminal
+ * operator. */
+void orc_compute_g12inter_direct(const orc_basis *outer, const orc_basis *inner, const double *Dmat,
+                                 int permuted, double drop, int ng, const double *gc, const double *gg,
+                                 int nthreads, double *Bout) {
+  const orc_gem gem = {ng, gc, gg};
+  coupling_build(outer, inner, Dmat, permuted, drop, nthreads, &gem, Bout, NULL);
+}
+
+static void coupling_build(const orc_basis *outer, const orc_basis *inner, const double *Dmat, int permuted,
+                           double drop, int nthreads, const orc_gem *gem, double *Bout, long *counts) {
   const int ns = outer->nshell, ons = inner->nshell;
   const int nA = orc_nbasis(outer), nB = orc_nbasis(inner);
   const int n = permuted ? nB : nA;
@@ -567,7 +682,7 @@ void orc_compute_coupling_direct(const orc_basis *outer, const orc_basis *inner,
           for (int os2 = os1; os2 < ons; ++os2) {
             if ((s1234++) % nthreads != tid) continue;
             const int on2 = ncart(inner->l[os2]);
-            orc_eri_shell(outer, s1, s2, inner, os1, os2, 0.0, buf);
+            shell_block(outer, s1, s2, inner, os1, os2, 0.0, gem, buf);
             for (int f1 = 0, f = 0; f1 < n1; ++f1) {
               const int bf1 = off[s1] + f1;
               for (int f2 = 0; f2 < n2; ++f2) {

@@ -60,6 +60,13 @@ def lib():
                                             C.c_double, dp]
         L.orc_compute_2body_all.restype = C.c_long
         L.orc_compute_coupling_all.argtypes = [C.c_void_p, C.c_void_p, C.c_double, dp]
+        L.orc_prim_g12.argtypes = [C.c_int] * 4 + [dp] * 4 + [C.c_double] * 4 + [C.c_int, dp, dp, dp]
+        L.orc_g12_shell.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, dp, dp, dp]
+        L.orc_compute_g12_direct.argtypes = [C.c_void_p, ip, ip, C.c_int, dp, C.c_double, C.c_int, dp, dp,
+                                             C.c_int, dp]
+        L.orc_compute_g12_direct.restype = C.c_long
+        L.orc_compute_g12inter_direct.argtypes = [C.c_void_p, C.c_void_p, dp, C.c_int, C.c_double, C.c_int, dp, dp,
+                                                  C.c_int, dp]
         _LIB = L
     return _LIB
 
@@ -255,3 +262,62 @@ def compute_coupling_directIT(this: Basis, other: Basis, id_this: int, id_other:
         return np.einsum("u,lsuv->vls", c, T)
     T = _coupling_tensor(this, other, drop)              # [u, v, l, s]
     return np.tensordot(c, T, axes=(0, 0))
+
+
+def _gem(coefficients, exponents):
+    c = np.ascontiguousarray(coefficients, dtype=np.float64).reshape(-1)
+    g = np.ascontiguousarray(exponents, dtype=np.float64).reshape(-1)
+    if c.shape != g.shape:
+        raise ValueError("geminal coefficients and exponents differ in length")
+    return len(c), (c if len(c) else np.zeros(1)), (g if len(g) else np.zeros(1))
+
+
+def prim_g12(la, lb, lc, ld, A, B, Cc, D, a, b, c, d, coefficients, exponents) -> np.ndarray:
+    """Unnormalised primitive integrals of sum_k c_k exp(-g_k r12^2), the
+    layout of prim_eri (McMurchie-Davidson with the geminal's Hermite table)."""
+    nc = lambda l: (l + 1) * (l + 2) // 2
+    out = np.zeros(nc(la) * nc(lb) * nc(lc) * nc(ld))
+    f = lambda x: np.ascontiguousarray(x, dtype=np.float64)
+    ng, gc, gg = _gem(coefficients, exponents)
+    lib().orc_prim_g12(la, lb, lc, ld, f(A), f(B), f(Cc), f(D), a, b, c, d, ng, gc, gg, out)
+    return out
+
+
+def g12_shell(b12: Basis, s1, s2, b34: Basis, s3, s4, coefficients, exponents) -> np.ndarray:
+    """Raw contracted geminal integrals of (s1 s2|s3 s4), libint order."""
+    n = int(b12.sizes[s1] * b12.sizes[s2] * b34.sizes[s3] * b34.sizes[s4])
+    out = np.zeros(n)
+    ng, gc, gg = _gem(coefficients, exponents)
+    lib().orc_g12_shell(b12.ptr, s1, s2, b34.ptr, s3, s4, ng, gc, gg, out)
+    return out
+
+
+def compute_g12_direct(b: Basis, D: np.ndarray, factor: float, coefficients, exponents, pairs=None,
+                       nthreads: int = 0) -> np.ndarray:
+    """LibintInterface::compute_g12_direct (Libint2Iface.cpp:2193-2365) with
+    the C wrapper's arguments (3687-3706): J + factor K with the contracted
+    Gaussian geminal, every quartet of the pair list, no density screen."""
+    if pairs is None:
+        pairs, _ = shellpair_list(b)
+    s1 = np.ascontiguousarray(pairs[:, 0], dtype=np.int32)
+    s2 = np.ascontiguousarray(pairs[:, 1], dtype=np.int32)
+    G = np.zeros(b.n * b.n)
+    ng, gc, gg = _gem(coefficients, exponents)
+    lib().orc_compute_g12_direct(b.ptr, s1, s2, len(s1), np.ascontiguousarray(D, dtype=np.float64).reshape(-1),
+                                 float(factor), ng, gc, gg, nthreads, G)
+    return G.reshape(b.n, b.n)
+
+
+def compute_g12inter_direct(this: Basis, other: Basis, id_this: int, id_other: int, D_other: np.ndarray,
+                            coefficients, exponents, drop: float = 1e-10, nthreads: int = 0) -> np.ndarray:
+    """libintinterface_compute_g12inter_direct (Libint2Iface.cpp:3708-3734,
+    2367-2550): the coupling build of compute_coupling_direct with the
+    geminal operator; same image convention (lower triangle, row-major)."""
+    permuted = id_this > id_other
+    outer, inner = (other, this) if permuted else (this, other)
+    n = this.n
+    B = np.zeros(n * n)
+    ng, gc, gg = _gem(coefficients, exponents)
+    lib().orc_compute_g12inter_direct(outer.ptr, inner.ptr, np.ascontiguousarray(D_other, dtype=np.float64).reshape(-1),
+                                      1 if permuted else 0, drop, ng, gc, gg, nthreads, B)
+    return B.reshape(n, n)
