@@ -306,6 +306,105 @@ __device__ __forceinline__ double quartet_dnorm(gcptr<double> Dn, int ns, const 
   return dn;
 }
 
+// The J/K updates of one quartet's raw block (out), normalised and weighted
+// by deg: J_cd, K_ac, K_bd, K_ad and K_bc go to G with FP64 atomics; J_ab is
+// added into jab (unscaled by deg and the bra norms) for the caller to reduce.
+// Libint2Iface.cpp:562-586 (exchange weight 0.5 * factor = half_factor).
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void jk_quartet_update(const double* out, const PairInfo& P, const PairInfo& K, int n,
+                                                  gcptr<double> D, gcptr<double> nrm, gptr<double> G,
+                                                  double deg, double half_factor, double* jab) {
+  constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  double na[NA], nb[NB], nc[NC], nd[ND];
+#pragma unroll
+  for (int i = 0; i < NA; ++i) na[i] = nrm[P.bfa + i];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) nb[i] = nrm[P.bfb + i];
+#pragma unroll
+  for (int i = 0; i < NC; ++i) nc[i] = nrm[K.bfa + i];
+#pragma unroll
+  for (int i = 0; i < ND; ++i) nd[i] = nrm[K.bfb + i];
+  // normalised density blocks
+  double dab[NA * NB], dcd[NC * ND], dac[NA * NC], dbd[NB * ND], dad[NA * ND], dbc[NB * NC];
+#pragma unroll
+  for (int i = 0; i < NA; ++i)
+#pragma unroll
+    for (int k = 0; k < NB; ++k) dab[i * NB + k] = D[(size_t)(P.bfa + i) * n + P.bfb + k] * na[i] * nb[k];
+#pragma unroll
+  for (int i = 0; i < NC; ++i)
+#pragma unroll
+    for (int k = 0; k < ND; ++k) dcd[i * ND + k] = D[(size_t)(K.bfa + i) * n + K.bfb + k] * nc[i] * nd[k];
+#pragma unroll
+  for (int i = 0; i < NA; ++i)
+#pragma unroll
+    for (int k = 0; k < NC; ++k) dac[i * NC + k] = D[(size_t)(P.bfa + i) * n + K.bfa + k] * na[i] * nc[k];
+#pragma unroll
+  for (int i = 0; i < NB; ++i)
+#pragma unroll
+    for (int k = 0; k < ND; ++k) dbd[i * ND + k] = D[(size_t)(P.bfb + i) * n + K.bfb + k] * nb[i] * nd[k];
+#pragma unroll
+  for (int i = 0; i < NA; ++i)
+#pragma unroll
+    for (int k = 0; k < ND; ++k) dad[i * ND + k] = D[(size_t)(P.bfa + i) * n + K.bfb + k] * na[i] * nd[k];
+#pragma unroll
+  for (int i = 0; i < NB; ++i)
+#pragma unroll
+    for (int k = 0; k < NC; ++k) dbc[i * NC + k] = D[(size_t)(P.bfb + i) * n + K.bfa + k] * nb[i] * nc[k];
+  double jcd[NC * ND], kac[NA * NC], kbd[NB * ND], kad[NA * ND], kbc[NB * NC];
+#pragma unroll
+  for (int k = 0; k < NC * ND; ++k) jcd[k] = 0.0;
+#pragma unroll
+  for (int k = 0; k < NA * NC; ++k) kac[k] = 0.0;
+#pragma unroll
+  for (int k = 0; k < NB * ND; ++k) kbd[k] = 0.0;
+#pragma unroll
+  for (int k = 0; k < NA * ND; ++k) kad[k] = 0.0;
+#pragma unroll
+  for (int k = 0; k < NB * NC; ++k) kbc[k] = 0.0;
+#pragma unroll
+  for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+    for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+      for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+        for (int id = 0; id < ND; ++id) {
+          const double v = out[((ia * NB + ib) * NC + ic) * ND + id];
+          jab[ia * NB + ib] += dcd[ic * ND + id] * v;
+          jcd[ic * ND + id] += dab[ia * NB + ib] * v;
+          kac[ia * NC + ic] += dbd[ib * ND + id] * v;
+          kbd[ib * ND + id] += dac[ia * NC + ic] * v;
+          kad[ia * ND + id] += dbc[ib * NC + ic] * v;
+          kbc[ib * NC + ic] += dad[ia * ND + id] * v;
+        }
+  const double sj = deg, sk = deg * half_factor;
+#pragma unroll
+  for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+    for (int id = 0; id < ND; ++id)
+      global_add(G + (size_t)(K.bfa + ic) * n + K.bfb + id, jcd[ic * ND + id] * sj * nc[ic] * nd[id]);
+#pragma unroll
+  for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+    for (int ic = 0; ic < NC; ++ic)
+      global_add(G + (size_t)(P.bfa + ia) * n + K.bfa + ic, kac[ia * NC + ic] * sk * na[ia] * nc[ic]);
+#pragma unroll
+  for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+    for (int id = 0; id < ND; ++id)
+      global_add(G + (size_t)(P.bfb + ib) * n + K.bfb + id, kbd[ib * ND + id] * sk * nb[ib] * nd[id]);
+#pragma unroll
+  for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+    for (int id = 0; id < ND; ++id)
+      global_add(G + (size_t)(P.bfa + ia) * n + K.bfb + id, kad[ia * ND + id] * sk * na[ia] * nd[id]);
+#pragma unroll
+  for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+    for (int ic = 0; ic < NC; ++ic)
+      global_add(G + (size_t)(P.bfb + ib) * n + K.bfa + ic, kbc[ib * NC + ic] * sk * nb[ib] * nc[ic]);
+}
+
 // ------------------------------------------------------------ J/K kernel
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
@@ -362,94 +461,7 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
                                          P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
     const double deg = (P.sa != P.sb ? 2.0 : 1.0) * (K.sa != K.sb ? 2.0 : 1.0) *
                        ((a.same_class && j == t.bra) ? 1.0 : 2.0);
-    double na[NA], nb[NB], nc[NC], nd[ND];
-#pragma unroll
-    for (int i = 0; i < NA; ++i) na[i] = nrm[P.bfa + i];
-#pragma unroll
-    for (int i = 0; i < NB; ++i) nb[i] = nrm[P.bfb + i];
-#pragma unroll
-    for (int i = 0; i < NC; ++i) nc[i] = nrm[K.bfa + i];
-#pragma unroll
-    for (int i = 0; i < ND; ++i) nd[i] = nrm[K.bfb + i];
-    // normalised density blocks
-    double dab[NA * NB], dcd[NC * ND], dac[NA * NC], dbd[NB * ND], dad[NA * ND], dbc[NB * NC];
-#pragma unroll
-    for (int i = 0; i < NA; ++i)
-#pragma unroll
-      for (int k = 0; k < NB; ++k) dab[i * NB + k] = D[(size_t)(P.bfa + i) * n + P.bfb + k] * na[i] * nb[k];
-#pragma unroll
-    for (int i = 0; i < NC; ++i)
-#pragma unroll
-      for (int k = 0; k < ND; ++k) dcd[i * ND + k] = D[(size_t)(K.bfa + i) * n + K.bfb + k] * nc[i] * nd[k];
-#pragma unroll
-    for (int i = 0; i < NA; ++i)
-#pragma unroll
-      for (int k = 0; k < NC; ++k) dac[i * NC + k] = D[(size_t)(P.bfa + i) * n + K.bfa + k] * na[i] * nc[k];
-#pragma unroll
-    for (int i = 0; i < NB; ++i)
-#pragma unroll
-      for (int k = 0; k < ND; ++k) dbd[i * ND + k] = D[(size_t)(P.bfb + i) * n + K.bfb + k] * nb[i] * nd[k];
-#pragma unroll
-    for (int i = 0; i < NA; ++i)
-#pragma unroll
-      for (int k = 0; k < ND; ++k) dad[i * ND + k] = D[(size_t)(P.bfa + i) * n + K.bfb + k] * na[i] * nd[k];
-#pragma unroll
-    for (int i = 0; i < NB; ++i)
-#pragma unroll
-      for (int k = 0; k < NC; ++k) dbc[i * NC + k] = D[(size_t)(P.bfb + i) * n + K.bfa + k] * nb[i] * nc[k];
-    double jcd[NC * ND], kac[NA * NC], kbd[NB * ND], kad[NA * ND], kbc[NB * NC];
-#pragma unroll
-    for (int k = 0; k < NC * ND; ++k) jcd[k] = 0.0;
-#pragma unroll
-    for (int k = 0; k < NA * NC; ++k) kac[k] = 0.0;
-#pragma unroll
-    for (int k = 0; k < NB * ND; ++k) kbd[k] = 0.0;
-#pragma unroll
-    for (int k = 0; k < NA * ND; ++k) kad[k] = 0.0;
-#pragma unroll
-    for (int k = 0; k < NB * NC; ++k) kbc[k] = 0.0;
-#pragma unroll
-    for (int ia = 0; ia < NA; ++ia)
-#pragma unroll
-      for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-        for (int ic = 0; ic < NC; ++ic)
-#pragma unroll
-          for (int id = 0; id < ND; ++id) {
-            const double v = out[((ia * NB + ib) * NC + ic) * ND + id];
-            jab[ia * NB + ib] += dcd[ic * ND + id] * v;
-            jcd[ic * ND + id] += dab[ia * NB + ib] * v;
-            kac[ia * NC + ic] += dbd[ib * ND + id] * v;
-            kbd[ib * ND + id] += dac[ia * NC + ic] * v;
-            kad[ia * ND + id] += dbc[ib * NC + ic] * v;
-            kbc[ib * NC + ic] += dad[ia * ND + id] * v;
-          }
-    const double sj = deg, sk = deg * M.half_factor;
-#pragma unroll
-    for (int ic = 0; ic < NC; ++ic)
-#pragma unroll
-      for (int id = 0; id < ND; ++id)
-        global_add(G + (size_t)(K.bfa + ic) * n + K.bfb + id, jcd[ic * ND + id] * sj * nc[ic] * nd[id]);
-#pragma unroll
-    for (int ia = 0; ia < NA; ++ia)
-#pragma unroll
-      for (int ic = 0; ic < NC; ++ic)
-        global_add(G + (size_t)(P.bfa + ia) * n + K.bfa + ic, kac[ia * NC + ic] * sk * na[ia] * nc[ic]);
-#pragma unroll
-    for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-      for (int id = 0; id < ND; ++id)
-        global_add(G + (size_t)(P.bfb + ib) * n + K.bfb + id, kbd[ib * ND + id] * sk * nb[ib] * nd[id]);
-#pragma unroll
-    for (int ia = 0; ia < NA; ++ia)
-#pragma unroll
-      for (int id = 0; id < ND; ++id)
-        global_add(G + (size_t)(P.bfa + ia) * n + K.bfb + id, kad[ia * ND + id] * sk * na[ia] * nd[id]);
-#pragma unroll
-    for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-      for (int ic = 0; ic < NC; ++ic)
-        global_add(G + (size_t)(P.bfb + ib) * n + K.bfa + ic, kbc[ib * NC + ic] * sk * nb[ib] * nc[ic]);
+    jk_quartet_update<LA, LB, LC, LD>(out, P, K, n, D, nrm, G, deg, M.half_factor, jab);
 #pragma unroll
     for (int k = 0; k < NA * NB; ++k) jab[k] *= deg;
   }

@@ -71,52 +71,45 @@ __device__ __forceinline__ void g12_quartet(const double* bprim, int nbp, const 
   C::template hrr<double>(acc, ABx, ABy, ABz, CDx, CDy, CDz, out);
 }
 
-// One lane per quartet (a wavefront: one bra pair, up to 64 kets); every
-// J/K update goes to Graw with a hardware FP64 atomic.
+// One lane per quartet (a wavefront: one bra pair, up to 64 kets), the J/K
+// updates of the Coulomb wavefront kernel: the bra pair's J block reduced
+// across the wavefront, the other five blocks as FP64 atomics.
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(256) void g12_jk_kernel(G12Args a) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   const int wid = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   if (wid >= a.ntasks) return;
   const Task t = a.tasks[wid];
-  const int j = t.j0 + (int)(threadIdx.x & 63);
-  if (j >= t.j1) return;
+  const int lane = threadIdx.x & 63;
+  const int j = t.j0 + lane;
+  const bool active = j < t.j1;
   const PairInfo P = a.bra[t.bra];
-  const PairInfo K = a.ket[j];
-  double out[NA * NB * NC * ND];
-  g12_quartet<LA, LB, LC, LD>(a.prim + (size_t)P.poff * kPrimPairStride, P.nprim,
-                              a.prim + (size_t)K.poff * kPrimPairStride, K.nprim, a.gem, P.ABx, P.ABy, P.ABz,
-                              K.ABx, K.ABy, K.ABz, out);
-  const double deg = (P.sa != P.sb ? 2.0 : 1.0) * (K.sa != K.sb ? 2.0 : 1.0) *
-                     ((a.same_class && j == t.bra) ? 1.0 : 2.0);
-  const size_t n = (size_t)a.n;
-  const gcptr<double> D = (gcptr<double>)a.D;
+  const PairInfo K = active ? a.ket[j] : P;
+  const gcptr<double> nrm = (gcptr<double>)a.norma;
   const gptr<double> G = (gptr<double>)a.Graw;
-  const double hf = a.half_factor;
-#pragma unroll 1
-  for (int ia = 0; ia < NA; ++ia) {
-    const size_t fa = P.bfa + ia;
-    const double wa = deg * a.norma[fa];
-    for (int ib = 0; ib < NB; ++ib) {
-      const size_t fb = P.bfb + ib;
-      const double wab = wa * a.norma[fb];
-      for (int ic = 0; ic < NC; ++ic) {
-        const size_t fc = K.bfa + ic;
-        const double wabc = wab * a.norma[fc];
+  double jab[NA * NB];
 #pragma unroll
-        for (int id = 0; id < ND; ++id) {
-          const size_t fd = K.bfb + id;
-          const double v = out[((ia * NB + ib) * NC + ic) * ND + id] * wabc * a.norma[fd];
-          global_add(G + fa * n + fb, D[fc * n + fd] * v);
-          global_add(G + fc * n + fd, D[fa * n + fb] * v);
-          global_add(G + fa * n + fc, hf * D[fb * n + fd] * v);
-          global_add(G + fb * n + fd, hf * D[fa * n + fc] * v);
-          global_add(G + fa * n + fd, hf * D[fb * n + fc] * v);
-          global_add(G + fb * n + fc, hf * D[fa * n + fd] * v);
-        }
-      }
-    }
+  for (int k = 0; k < NA * NB; ++k) jab[k] = 0.0;
+  if (active) {
+    double out[NA * NB * NC * ND];
+    g12_quartet<LA, LB, LC, LD>(a.prim + (size_t)P.poff * kPrimPairStride, P.nprim,
+                                a.prim + (size_t)K.poff * kPrimPairStride, K.nprim, a.gem, P.ABx, P.ABy, P.ABz,
+                                K.ABx, K.ABy, K.ABz, out);
+    const double deg = (P.sa != P.sb ? 2.0 : 1.0) * (K.sa != K.sb ? 2.0 : 1.0) *
+                       ((a.same_class && j == t.bra) ? 1.0 : 2.0);
+    jk_quartet_update<LA, LB, LC, LD>(out, P, K, a.n, (gcptr<double>)a.D, nrm, G, deg, a.half_factor, jab);
+#pragma unroll
+    for (int k = 0; k < NA * NB; ++k) jab[k] *= deg;
   }
+  const uint64_t amask = __ballot(active);
+#pragma unroll
+  for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+    for (int ib = 0; ib < NB; ++ib) {
+      const double s = wave_sum(jab[ia * NB + ib]);
+      if (lane == 0 && amask)
+        global_add(G + (size_t)(P.bfa + ia) * a.n + P.bfb + ib, s * nrm[P.bfa + ia] * nrm[P.bfb + ib]);
+    }
 }
 
 template <int LA, int LB, int LC, int LD>

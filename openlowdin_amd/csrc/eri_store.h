@@ -25,7 +25,7 @@ namespace lowdin {
 constexpr int kStoreFullUnroll = 216;
 
 // ------------------------------------------------- intra-species store / IT
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, int MODE>
 __global__ __launch_bounds__(256) void store_kernel(StoreArgs a) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   // Element loops: fully unrolled for small blocks; for large ones (the
@@ -41,7 +41,11 @@ __global__ __launch_bounds__(256) void store_kernel(StoreArgs a) {
   const PairInfo P = a.bra[t.bra];
   const PairInfo K = a.ket[j];
   if (a.screen && quartet_dnorm((gcptr<double>)a.Dn, a.nshell, P, K) * P.Q * K.Q < a.eps) return;
-  double out[NA * NB * NC * ND];
+  // the raw block: private memory, or for the largest class this thread's
+  // slot of the library's staging buffer (written and read by this lane only)
+  constexpr int NQ = NA * NB * NC * ND;
+  double priv[NQ >= kStoreStagedNQ ? 1 : NQ];
+  double* out = NQ >= kStoreStagedNQ ? a.blk + ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * NQ : priv;
   contracted_quartet<LA, LB, LC, LD>(a.prim + (size_t)P.poff * kPrimPairStride, P.nprim,
                                      a.prim + (size_t)K.poff * kPrimPairStride, K.nprim, a.boys,
                                      P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
@@ -54,21 +58,16 @@ __global__ __launch_bounds__(256) void store_kernel(StoreArgs a) {
   for (int i = 0; i < NC; ++i) nc[i] = a.norma[K.bfa + i];
 #pragma unroll
   for (int i = 0; i < ND; ++i) nd[i] = a.norma[K.bfb + i];
-  // drop (|raw| > drop kept) and normalise; a dropped integral is 0 here
-#pragma unroll U1
-  for (int ia = 0; ia < NA; ++ia)
-#pragma unroll U1
-    for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-      for (int ic = 0; ic < NC; ++ic)
-#pragma unroll
-        for (int id = 0; id < ND; ++id) {
-          const int e = ((ia * NB + ib) * NC + ic) * ND + id;
-          const double v = out[e];
-          out[e] = fabs(v) > a.drop ? v * na[ia] * nb[ib] * nc[ic] * nd[id] : 0.0;
-        }
+  // drop (|raw| > drop kept) and normalise, applied where each element is
+  // read (the raw block is not rewritten: a second pass over it would keep
+  // it live in private memory beside the recursion's own)
+  const double drop = a.drop;
+  auto kept = [&](int ia, int ib, int ic, int id) {
+    const double v = out[((ia * NB + ib) * NC + ic) * ND + id];
+    return fabs(v) > drop ? v * na[ia] * nb[ib] * nc[ic] * nd[id] : 0.0;
+  };
   const size_t n = (size_t)a.n, n2 = n * n;
-  if (a.mode == 0) {
+  if constexpr (MODE == 0) {
     // All eight images of every kept integral.  Images that coincide are
     // written by this lane only, so the result does not depend on timing.
     double* A = a.out;
@@ -81,7 +80,7 @@ __global__ __launch_bounds__(256) void store_kernel(StoreArgs a) {
         for (int ic = 0; ic < NC; ++ic)
 #pragma unroll
           for (int id = 0; id < ND; ++id) {
-            const double v = out[((ia * NB + ib) * NC + ic) * ND + id];
+            const double v = kept(ia, ib, ic, id);
             if (v == 0.0) continue;
             const size_t i = P.bfa + ia, jj = P.bfb + ib, k = K.bfa + ic, l = K.bfb + id;
             A[i * n3 + jj * n2 + k * n + l] = v;
@@ -93,34 +92,17 @@ __global__ __launch_bounds__(256) void store_kernel(StoreArgs a) {
             A[l * n3 + k * n2 + i * n + jj] = v;
             A[l * n3 + k * n2 + jj * n + i] = v;
           }
-    return;
-  }
-  // First-quarter transform.  Each distinct shell-level permutation of the
-  // quartet contributes once; within a pair of one shell the block already
-  // holds both orders, so only its k <= l elements feed a (k, l) slot.
-  const gptr<double> GG = (gptr<double>)a.out;
-  const gcptr<double> c = (gcptr<double>)a.c;
-  const bool sab = P.sa != P.sb, scd = K.sa != K.sb;
-  const bool spk = !(a.same_class && j == t.bra);
-  // (ab|cd): i in A, j in B, {k, l} = {c, d}
+  } else {
+    // First-quarter transform.  Each distinct shell-level permutation of the
+    // quartet contributes once; within a pair of one shell the block already
+    // holds both orders, so only its k <= l elements feed a (k, l) slot.
+    const gptr<double> GG = (gptr<double>)a.out;
+    const gcptr<double> c = (gcptr<double>)a.c;
+    const bool sab = P.sa != P.sb, scd = K.sa != K.sb;
+    const bool spk = !(a.same_class && j == t.bra);
+    // (ab|cd): i in A, j in B, {k, l} = {c, d}
 #pragma unroll U1
-  for (int ib = 0; ib < NB; ++ib)
-#pragma unroll U1
-    for (int ic = 0; ic < NC; ++ic)
-#pragma unroll
-      for (int id = 0; id < ND; ++id) {
-        const int fc = K.bfa + ic, fd = K.bfb + id;
-        if (!scd && fc > fd) continue;
-        double s = 0.0;
-#pragma unroll
-        for (int ia = 0; ia < NA; ++ia) s += c[P.bfa + ia] * out[((ia * NB + ib) * NC + ic) * ND + id];
-        if (s != 0.0)
-          global_add(GG + (size_t)(P.bfb + ib) * n2 + (size_t)min(fc, fd) * n + max(fc, fd), s);
-      }
-  // (ba|cd)
-  if (sab) {
-#pragma unroll U1
-    for (int ia = 0; ia < NA; ++ia)
+    for (int ib = 0; ib < NB; ++ib)
 #pragma unroll U1
       for (int ic = 0; ic < NC; ++ic)
 #pragma unroll
@@ -129,31 +111,31 @@ __global__ __launch_bounds__(256) void store_kernel(StoreArgs a) {
           if (!scd && fc > fd) continue;
           double s = 0.0;
 #pragma unroll
-          for (int ib = 0; ib < NB; ++ib) s += c[P.bfb + ib] * out[((ia * NB + ib) * NC + ic) * ND + id];
+          for (int ia = 0; ia < NA; ++ia) s += c[P.bfa + ia] * kept(ia, ib, ic, id);
           if (s != 0.0)
-            global_add(GG + (size_t)(P.bfa + ia) * n2 + (size_t)min(fc, fd) * n + max(fc, fd), s);
+            global_add(GG + (size_t)(P.bfb + ib) * n2 + (size_t)min(fc, fd) * n + max(fc, fd), s);
         }
-  }
-  if (!spk) return;
-  // (cd|ab)
+    // (ba|cd)
+    if (sab) {
 #pragma unroll U1
-  for (int id = 0; id < ND; ++id)
+      for (int ia = 0; ia < NA; ++ia)
 #pragma unroll U1
-    for (int ia = 0; ia < NA; ++ia)
+        for (int ic = 0; ic < NC; ++ic)
 #pragma unroll
-      for (int ib = 0; ib < NB; ++ib) {
-        const int fa = P.bfa + ia, fb = P.bfb + ib;
-        if (!sab && fa > fb) continue;
-        double s = 0.0;
+          for (int id = 0; id < ND; ++id) {
+            const int fc = K.bfa + ic, fd = K.bfb + id;
+            if (!scd && fc > fd) continue;
+            double s = 0.0;
 #pragma unroll
-        for (int ic = 0; ic < NC; ++ic) s += c[K.bfa + ic] * out[((ia * NB + ib) * NC + ic) * ND + id];
-        if (s != 0.0)
-          global_add(GG + (size_t)(K.bfb + id) * n2 + (size_t)min(fa, fb) * n + max(fa, fb), s);
-      }
-  // (dc|ab)
-  if (scd) {
+            for (int ib = 0; ib < NB; ++ib) s += c[P.bfb + ib] * kept(ia, ib, ic, id);
+            if (s != 0.0)
+              global_add(GG + (size_t)(P.bfa + ia) * n2 + (size_t)min(fc, fd) * n + max(fc, fd), s);
+          }
+    }
+    if (!spk) return;
+    // (cd|ab)
 #pragma unroll U1
-    for (int ic = 0; ic < NC; ++ic)
+    for (int id = 0; id < ND; ++id)
 #pragma unroll U1
       for (int ia = 0; ia < NA; ++ia)
 #pragma unroll
@@ -162,10 +144,27 @@ __global__ __launch_bounds__(256) void store_kernel(StoreArgs a) {
           if (!sab && fa > fb) continue;
           double s = 0.0;
 #pragma unroll
-          for (int id = 0; id < ND; ++id) s += c[K.bfb + id] * out[((ia * NB + ib) * NC + ic) * ND + id];
+          for (int ic = 0; ic < NC; ++ic) s += c[K.bfa + ic] * kept(ia, ib, ic, id);
           if (s != 0.0)
-            global_add(GG + (size_t)(K.bfa + ic) * n2 + (size_t)min(fa, fb) * n + max(fa, fb), s);
+            global_add(GG + (size_t)(K.bfb + id) * n2 + (size_t)min(fa, fb) * n + max(fa, fb), s);
         }
+    // (dc|ab)
+    if (scd) {
+#pragma unroll U1
+      for (int ic = 0; ic < NC; ++ic)
+#pragma unroll U1
+        for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+          for (int ib = 0; ib < NB; ++ib) {
+            const int fa = P.bfa + ia, fb = P.bfb + ib;
+            if (!sab && fa > fb) continue;
+            double s = 0.0;
+#pragma unroll
+            for (int id = 0; id < ND; ++id) s += c[K.bfb + id] * kept(ia, ib, ic, id);
+            if (s != 0.0)
+              global_add(GG + (size_t)(K.bfa + ic) * n2 + (size_t)min(fa, fb) * n + max(fa, fb), s);
+          }
+    }
   }
 }
 
@@ -185,7 +184,11 @@ __global__ __launch_bounds__(256) void coupling_store_kernel(CouplingStoreArgs a
   if (j >= t.j1) return;
   const PairInfo P = a.bra[t.bra];
   const PairInfo K = a.ket[j];
-  double out[NA * NB * NC * ND];
+  // the raw block: private memory, or for the largest class this thread's
+  // slot of the library's staging buffer (written and read by this lane only)
+  constexpr int NQ = NA * NB * NC * ND;
+  double priv[NQ >= kStoreStagedNQ ? 1 : NQ];
+  double* out = NQ >= kStoreStagedNQ ? a.blk + ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * NQ : priv;
   contracted_quartet<LA, LB, LC, LD>(a.bprim + (size_t)P.poff * kPrimPairStride, P.nprim,
                                      a.kprim + (size_t)K.poff * kPrimPairStride, K.nprim, a.boys,
                                      P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
@@ -310,7 +313,10 @@ template <int LA, int LB, int LC, int LD>
 static void launch_store_t(const StoreArgs& a, hipStream_t s) {
   if (a.ntasks <= 0) return;
   const unsigned blocks = (unsigned)((a.ntasks + 3) / 4);
-  hipLaunchKernelGGL((store_kernel<LA, LB, LC, LD>), dim3(blocks), dim3(256), 0, s, a);
+  if (a.mode == 0)
+    hipLaunchKernelGGL((store_kernel<LA, LB, LC, LD, 0>), dim3(blocks), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((store_kernel<LA, LB, LC, LD, 1>), dim3(blocks), dim3(256), 0, s, a);
 }
 
 template <int LA, int LB, int LC, int LD>

@@ -210,7 +210,7 @@ struct Species {
   DevBuf<double> d_D, d_G, d_Dother, d_R;
   TaskSet store_ts;                  // wavefront tasks of the integral store / transform
   uint64_t store_gen = ~0ull;        // sig_gen the store tasks were built from
-  DevBuf<double> d_Dn1, d_cvec, d_store, d_Graw1;
+  DevBuf<double> d_Dn1, d_cvec, d_store, d_Graw1, d_blk;
   lowdin_fock_stats stats{};
   int mixed = 0;            // mixed precision: FP32 quartets below tau
   double tau = 0.0;
@@ -662,25 +662,43 @@ static void ensure_launch_events(BuildPlan& p) {
 static inline bool heavy_launch(const TaskSet::Launch& L) { return L.la + L.lb + L.lc + L.ld >= kHeavyL; }
 
 // Class launches of one build on the side streams, forked from and joined
-// back to st: largest first, round robin over the side streams so that small
-// latency-bound classes overlap the large ones; the scratch-heavy classes
-// always on side stream 0.  which: 0 all, 1 all but the scratch-heavy ones,
-// 2 only the scratch-heavy ones.
-static void enqueue_classes(BuildPlan& p, hipStream_t st, bool timed, int which) {
+// back to st.  The scratch-heavy classes (few quartets, long per-lane
+// latency) go first, all on side stream 0, so that they run beside the
+// large classes instead of after them; the other classes follow largest
+// first, round robin over the remaining side streams (all four when the
+// plan has no heavy class).  which: 0 all, 1 all but the scratch-heavy
+// ones, 2 only the scratch-heavy ones.
+static void join_side_streams(hipStream_t st) {
+  Context& c = ctx();
+  for (int q = 0; q < kSideStreams; ++q) {
+    HIPCHK(hipEventRecord(c.join[q], c.side[q]));
+    HIPCHK(hipStreamWaitEvent(st, c.join[q], 0));
+  }
+}
+
+static void enqueue_classes(BuildPlan& p, hipStream_t st, bool timed, int which, bool join = true) {
   Context& c = ctx();
   const size_t nl = p.ts.launches.size();
-  bool any = false;
-  for (size_t k = 0; k < nl; ++k)
+  bool any = false, any_heavy = false;
+  for (size_t k = 0; k < nl; ++k) {
     any = any || which == 0 || (which == 1) != heavy_launch(p.ts.launches[k]);
+    any_heavy = any_heavy || heavy_launch(p.ts.launches[k]);
+  }
   if (!any) return;
   HIPCHK(hipEventRecord(c.fork, st));
   for (int q = 0; q < kSideStreams; ++q) HIPCHK(hipStreamWaitEvent(c.side[q], c.fork, 0));
-  for (size_t r = 0; r < nl; ++r) {
+  const int first_light = (any_heavy && which == 0) ? 1 : 0;
+  size_t nlight = 0;
+  for (size_t pass = 0; pass < 2 * nl; ++pass) {
+    const size_t r = pass % nl;
     const size_t k = p.ts.order[r];
     const TaskSet::Launch& L = p.ts.launches[k];
     const bool heavy = heavy_launch(L);
+    if ((pass < nl) != heavy) continue;   // first pass: heavy classes, second: the rest
     if ((which == 1 && heavy) || (which == 2 && !heavy)) continue;
-    hipStream_t ls = c.side[(p.serialize || heavy) ? 0 : r % kSideStreams];
+    int q = 0;
+    if (!p.serialize && !heavy) q = first_light + (int)(nlight++ % (size_t)(kSideStreams - first_light));
+    hipStream_t ls = c.side[q];
     JKArgs a{};
     a.tasks = p.ts.d_tasks.p + L.off;
     a.ntasks = (int)L.count;
@@ -698,10 +716,7 @@ static void enqueue_classes(BuildPlan& p, hipStream_t st, bool timed, int which)
     if (!ok) throw std::runtime_error("no J/K kernel for class");
     if (timed) HIPCHK(hipEventRecord(p.launch_ev[2 * k + 1], ls));
   }
-  for (int q = 0; q < kSideStreams; ++q) {
-    HIPCHK(hipEventRecord(c.join[q], c.side[q]));
-    HIPCHK(hipStreamWaitEvent(st, c.join[q], 0));
-  }
+  if (join) join_side_streams(st);
 }
 
 // Dnorm and zeroed G_raw (no memset: capturable).
@@ -769,17 +784,17 @@ static void enqueue_plan(BuildPlan& p, const double* dD, double* dG, const doubl
   if (timed) ensure_launch_events(p);
   HIPCHK(hipEventRecord(c.ev0, st));
   if (p.use_graph) {
-    // The graph holds the prologue and the classes without a large private
-    // segment.  Kernels in a graph run on the runtime's own streams (one per
-    // branch), so the scratch-heavy classes stay outside it, on side
-    // stream 0, with the epilogue.
+    // The graph holds the classes without a large private segment.  Kernels
+    // in a graph run on the runtime's own streams (one per branch), so the
+    // scratch-heavy classes stay outside it, on side stream 0: they are
+    // enqueued after the prologue and before the graph launch, and joined
+    // with it before the epilogue, so that they run beside the graph.
     if (!p.graph || p.graph_layout != p.layout || p.graph_stream != st || p.graph_any_dn != p.any_dn) {
       if (p.graph) HIPCHK(hipGraphExecDestroy(p.graph));
       p.graph = nullptr;
       hipGraph_t g = nullptr;
       HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       try {
-        enqueue_prologue(p, st);
         enqueue_classes(p, st, false, 1);
       } catch (...) {
         (void)hipStreamEndCapture(st, &g);
@@ -794,8 +809,10 @@ static void enqueue_plan(BuildPlan& p, const double* dD, double* dG, const doubl
       p.graph_stream = st;
       p.graph_any_dn = p.any_dn;
     }
+    enqueue_prologue(p, st);
+    enqueue_classes(p, st, false, 2, false);
     HIPCHK(hipGraphLaunch(p.graph, st));
-    enqueue_classes(p, st, false, 2);
+    join_side_streams(st);
     enqueue_epilogue(p, st);
   } else {
     enqueue_prologue(p, st);
@@ -973,6 +990,26 @@ static TaskSet& store_taskset(Species& s, hipStream_t st) {
   return ts;
 }
 
+// Launch one store class; the classes whose raw block is staged in HBM
+// (kStoreStagedNQ integrals per quartet) go in chunks of kStoreStagedTasks
+// tasks over the species' staging buffer, one after another on hs.
+template <typename Args, typename Dispatch>
+static void launch_store_class(const TaskSet::Launch& L, Args g, Species& s, hipStream_t hs, Dispatch dispatch) {
+  const int nq = ncart_h(L.la) * ncart_h(L.lb) * ncart_h(L.lc) * ncart_h(L.ld);
+  if (nq < kStoreStagedNQ) {
+    if (!dispatch(L.la, L.lb, L.lc, L.ld, g, hs)) throw std::runtime_error("no store kernel for class");
+    return;
+  }
+  const Task* tasks = g.tasks;
+  const int ntasks = g.ntasks;
+  g.blk = s.d_blk.ensure((size_t)kStoreStagedTasks * 64 * nq);
+  for (int t0 = 0; t0 < ntasks; t0 += kStoreStagedTasks) {
+    g.tasks = tasks + t0;
+    g.ntasks = std::min(kStoreStagedTasks, ntasks - t0);
+    if (!dispatch(L.la, L.lb, L.lc, L.ld, g, hs)) throw std::runtime_error("no store kernel for class");
+  }
+}
+
 // Candidate quartets / ERIs of a task range (before the density screen).
 static void count_tasks(const TaskSet& ts, const TaskSet::Launch& L, const PairInfo* bras, const PairInfo* kets,
                         lowdin_fock_stats* stt) {
@@ -1022,7 +1059,7 @@ static void enqueue_store(Species& s, const double* dD, const double* dC, double
     g.mode = mode;
     g.eps = 2.220446049250313080847e-16;  // DBL_EPSILON, Libint2Iface.h:273-274, 316-317
     g.drop = 1.0e-10;                      // Libint2Iface.cpp:794, 2701
-    if (!dispatch_store(L.la, L.lb, L.lc, L.ld, g, hs)) throw std::runtime_error("no store kernel for class");
+    launch_store_class(L, g, s, hs, dispatch_store);
     count_tasks(ts, L, s.sig.pairs.data() + s.sig.off[L.cbra], s.sig.pairs.data() + s.sig.off[L.cket], stt);
   }
   HIPCHK(hipGetLastError());
@@ -1081,7 +1118,7 @@ static void enqueue_coupling_store(Species& a, Species& b, const double* dC, dou
     g.sx2 = nb * nb;
     g.sy1 = nb;
     g.drop = 1.0e-10;  // Libint2Iface.cpp:1456, 2865
-    if (!dispatch_cstore(L.la, L.lb, L.lc, L.ld, g, hs)) throw std::runtime_error("no coupling store kernel for class");
+    launch_store_class(L, g, a, hs, dispatch_cstore);
     count_tasks(*ts, L, sb.all.pairs.data() + sb.all.off[L.cbra], sk.all.pairs.data() + sk.all.off[L.cket], stt);
   }
   HIPCHK(hipGetLastError());

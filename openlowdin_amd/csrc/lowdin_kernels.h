@@ -69,6 +69,15 @@ struct CouplingArgs {
   double drop;             // raw-integral drop (1e-10)
 };
 
+// Store kernels of classes with at least this many integrals per quartet
+// ((dd|dd)) keep the raw block in a buffer in HBM owned by the library (one
+// block per launched thread, StoreArgs::blk) instead of private memory, so
+// that their private segment stays within that of the Fock-build kernels;
+// launches of those classes are split into chunks of at most this many
+// tasks (256 tasks: 16384 threads, 170 MB of staging).
+constexpr int kStoreStagedNQ = 1296;
+constexpr int kStoreStagedTasks = 256;
+
 // Integral store (mode 0) or first-quarter transform (mode 1) over the
 // intra-species quartets of one species.
 //   mode 0: A[i][j][k][l] (n^4) = (ij|kl) * norma^4 for |raw| > drop, all
@@ -86,6 +95,7 @@ struct StoreArgs {
   const double* c;         // mode 1: row p of the coefficient matrix (n)
   double* out;
   const double* boys;
+  double* blk;             // classes with kStoreStagedNQ integrals per block: one raw block per thread
   int n;
   int nshell;
   int same_class;          // bra and ket lists are the same class list
@@ -114,6 +124,7 @@ struct CouplingStoreArgs {
   const double* c;         // mode 1: row p of T's coefficient matrix
   double* out;
   const double* boys;
+  double* blk;             // as StoreArgs::blk
   int bra_is_x;            // mode 0: bra pair belongs to X
   int bra_is_t;            // mode 1: bra pair belongs to T
   int mode;
