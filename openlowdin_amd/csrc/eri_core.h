@@ -105,14 +105,22 @@ __device__ __forceinline__ void boys_f64(double T, TP tab, double* F) {
       for (int m = L - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + eT) * (1.0 / (2 * m + 1));
     }
   } else {
-    // erf(sqrt(T)) = 1 to 4e-19 here; the upward recursion keeps its
-    // exp(-T) term, which is not negligible against F_m for m ~ T/5.
-    F[0] = 0.88622692545275801365 * rsqrt_nr(T);
+    // erf(sqrt(T)) = 1 to 4e-19 here, and 1/(2T) = r^2 / 2.  The upward
+    // recursion keeps its exp(-T) term, which is not negligible against F_m
+    // for m ~ T/5; for L <= 2 it is below F_L's rounding (e^-40 = 4.2e-18
+    // against F_2(40) = 6.6e-5: 8e-16 relative) and is left out.
+    const double r = rsqrt_nr(T);
+    F[0] = 0.88622692545275801365 * r;
     if (L > 0) {
-      const double o2t = 0.5 / T;
-      const double eT = exp(-T);
+      const double o2t = 0.5 * r * r;
+      if constexpr (L <= 2) {
 #pragma unroll
-      for (int m = 0; m < L; ++m) F[m + 1] = ((double)(2 * m + 1) * F[m] - eT) * o2t;
+        for (int m = 0; m < L; ++m) F[m + 1] = (double)(2 * m + 1) * F[m] * o2t;
+      } else {
+        const double eT = exp(-T);
+#pragma unroll
+        for (int m = 0; m < L; ++m) F[m + 1] = ((double)(2 * m + 1) * F[m] - eT) * o2t;
+      }
     }
   }
 }
