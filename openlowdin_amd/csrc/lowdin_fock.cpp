@@ -363,20 +363,44 @@ static void build_pairset(const Species& s, const std::vector<std::pair<int, int
   ps.q_ready = false;
 }
 
+// Morton (Z-order) key of a point: coordinates quantised on a 0.5 bohr grid
+// from the box corner lo, 21 bits per axis interleaved.
+static uint64_t morton_key(const double* P, const double* lo) {
+  uint64_t key = 0;
+  uint32_t c[3];
+  for (int d = 0; d < 3; ++d) {
+    const double q = std::max(0.0, (P[d] - lo[d]) * 2.0);
+    c[d] = (uint32_t)std::min(q, 2097151.0);
+  }
+  for (int b = 20; b >= 0; --b)
+    for (int d = 0; d < 3; ++d) key = (key << 1) | ((c[d] >> b) & 1u);
+  return key;
+}
+
 // After Q is known: within each pair class order pairs by primitive count
-// (descending) then Q (descending), so that lanes of a wavefront run equal
-// primitive loops and the FP32 (small-bound) quartets of mixed precision
-// form contiguous runs.  Rewrites the primitive-pair records in that order.
+// (descending), so that lanes of a wavefront run equal primitive loops, then
+// along a Z-order curve of the pair centres (first primitive pair's P), so
+// that the 64 kets of a wavefront lie close together: for a distant bra
+// every lane then takes the Boys function's asymptotic branch and the
+// wavefront skips the table branch.  Rewrites the primitive-pair records in
+// that order.
 static void resort_by_q(PairSet& ps, hipStream_t st) {
   std::vector<size_t> idx(ps.pairs.size());
   for (size_t k = 0; k < idx.size(); ++k) idx[k] = k;
+  double lo[3] = {1e300, 1e300, 1e300};
+  std::vector<uint64_t> key(ps.pairs.size(), 0);
+  for (const PairInfo& P : ps.pairs)
+    if (P.nprim > 0)
+      for (int d = 0; d < 3; ++d) lo[d] = std::min(lo[d], ps.prims[(size_t)P.poff * kPrimPairStride + 2 + d]);
+  for (size_t k = 0; k < ps.pairs.size(); ++k)
+    if (ps.pairs[k].nprim > 0) key[k] = morton_key(&ps.prims[(size_t)ps.pairs[k].poff * kPrimPairStride + 2], lo);
   std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) {
     const PairInfo& a = ps.pairs[x];
     const PairInfo& b = ps.pairs[y];
     const int ca = pair_class(a.la, a.lb), cb = pair_class(b.la, b.lb);
     if (ca != cb) return ca < cb;
     if (a.nprim != b.nprim) return a.nprim > b.nprim;
-    return a.Q > b.Q;
+    return key[x] < key[y];
   });
   std::vector<PairInfo> pairs;
   std::vector<double> prims;
