@@ -213,6 +213,7 @@ struct Species {
   DevBuf<double> d_Dn1, d_cvec, d_store, d_Graw1, d_blk;
   lowdin_fock_stats stats{};
   int mixed = 0;            // mixed precision: FP32 quartets below tau
+  int sig_mixed = 0;        // the precision mode the pair order of sig was chosen for
   double tau = 0.0;
 
   ~Species();
@@ -379,12 +380,15 @@ static uint64_t morton_key(const double* P, const double* lo) {
 
 // After Q is known: within each pair class order pairs by primitive count
 // (descending), so that lanes of a wavefront run equal primitive loops, then
-// along a Z-order curve of the pair centres (first primitive pair's P), so
-// that the 64 kets of a wavefront lie close together: for a distant bra
-// every lane then takes the Boys function's asymptotic branch and the
-// wavefront skips the table branch.  Rewrites the primitive-pair records in
-// that order.
-static void resort_by_q(PairSet& ps, hipStream_t st) {
+//   FP64 builds: along a Z-order curve of the pair centres (first primitive
+//     pair's P), so that the 64 kets of a wavefront lie close together: for
+//     a distant bra every lane then takes the Boys function's asymptotic
+//     branch and the wavefront skips the table branch;
+//   mixed precision (by_q): by decade of the normalised bound Qn
+//     (descending), so that the FP32 (small-bound) quartets form contiguous
+//     runs of kets, then along the Z-order curve within a decade.
+// Rewrites the primitive-pair records in that order.
+static void resort_by_q(PairSet& ps, hipStream_t st, bool by_q) {
   std::vector<size_t> idx(ps.pairs.size());
   for (size_t k = 0; k < idx.size(); ++k) idx[k] = k;
   double lo[3] = {1e300, 1e300, 1e300};
@@ -400,6 +404,11 @@ static void resort_by_q(PairSet& ps, hipStream_t st) {
     const int ca = pair_class(a.la, a.lb), cb = pair_class(b.la, b.lb);
     if (ca != cb) return ca < cb;
     if (a.nprim != b.nprim) return a.nprim > b.nprim;
+    if (by_q) {
+      const int da = (int)std::floor(std::log10(std::max(a.Qn, 1e-300)));
+      const int db = (int)std::floor(std::log10(std::max(b.Qn, 1e-300)));
+      if (da != db) return da > db;
+    }
     return key[x] < key[y];
   });
   std::vector<PairInfo> pairs;
@@ -474,7 +483,8 @@ static void ensure_sig(Species& s, hipStream_t st) {
   if (!s.sig.valid) {
     build_pairset(s, s.pairlist, s.sig, st);
     compute_schwarz(s.sig, s.d_norma.p, st);
-    resort_by_q(s.sig, st);
+    resort_by_q(s.sig, st, s.mixed != 0);
+    s.sig_mixed = s.mixed;
     s.sig_gen++;
   }
 }
@@ -1703,6 +1713,8 @@ int lowdin_fock_set_precision(void* lint, int mode, double tau) {
   }
   s->mixed = mode;
   s->tau = tau;
+  // the pair order depends on the mode: re-sort on the next build
+  if (s->sig.valid && s->sig_mixed != mode) s->sig.valid = false;
   lowdin::g_last_error.clear();
   return 0;
 }
