@@ -313,6 +313,7 @@ def main():
     ap.add_argument("--serving-jobs", type=int, default=16)
     ap.add_argument("--serving-rates", default="0.5,1,2")
     ap.add_argument("--no-store", action="store_true", help="skip the integral store / transform runs")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default per-class kernel shapes")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 256 MiB device-to-device copy (profiler byte-counter calibration)")
     args = ap.parse_args()
@@ -341,6 +342,12 @@ def main():
     dD = H.DeviceArray.from_host(np.asfortranarray(D).ravel(order="F"))
     dG = H.DeviceArray((n * n,))
     for _ in range(args.warmup):
+        h.compute_2body_device(dD.ptr, dG.ptr, -0.5, stream.handle, synchronize=True)
+    tuned = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        h.autotune(dD.ptr, dG.ptr, -0.5, stream.handle, reps=1)
+        tuned = {"seconds": round(time.perf_counter() - t0, 2), "targets": h.targets()}
         h.compute_2body_device(dD.ptr, dG.ptr, -0.5, stream.handle, synchronize=True)
     st = h.stats()
     print("[bench] warm: %d quartets, %d ERIs, %.2f ms kernels" % (st["quartets"], st["eris"], st["kernel_ms"]),
@@ -426,6 +433,7 @@ def main():
                      "traffic_source": "profiles/jk_c20h42_fp64_traffic.md" if traffic else None, "flops_per_launch": fl,
                      "avg_launch_ms": round(avg_ms, 4),
                      "timing": "HIP events around the kernel, class launches serialised (%d builds)" % nprof},
+        "autotune": tuned,
         "serialized_ms_per_build": round(sum(v[0] for v in per_class.values()) / nprof, 4),
         "classes": classes,
     }

@@ -275,6 +275,12 @@ int lowdin_fock_batch_autotune(void* batch, const double* d_dens, double* d_resu
  * strip kernel's workgroup target.  Returns 21. */
 int lowdin_fock_batch_targets(void* batch, int* targets);
 
+/* The same autotuner and choices for one species' own build (the plan behind
+ * LibintInterface_compute_2body_direct and lowdin_fock_compute_2body_device):
+ * d_dens / d_result n x n device buffers, `factor` the exchange factor. */
+int lowdin_fock_autotune(void* lint, const double* d_dens, double* d_result, double factor, void* stream, int reps);
+int lowdin_fock_targets(void* lint, int* targets);
+
 /* ------------------------------------------------ device-resident SCF
  * The caller side of the path on the GPU: a multi-species Hartree-Fock
  * iteration whose matrices stay in HBM (openLOWDIN src/scf/MultiSCF.f90:

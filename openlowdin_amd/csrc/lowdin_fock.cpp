@@ -258,7 +258,7 @@ struct BuildPlan {
 
   BuildPlan() {
     for (int k = 0; k < kNumClassPairs; ++k) {
-      target_wg[k] = 2048;
+      target_wg[k] = 512;
       kind[k] = 0;
     }
   }
@@ -1998,6 +1998,24 @@ int lowdin_fock_batch_autotune(void* batch, const double* d_dens, double* d_resu
     fprintf(stderr, "lowdin_fock: %s\n", lowdin::g_last_error.c_str());
     return -1;
   }
+}
+
+int lowdin_fock_autotune(void* lint, const double* d_dens, double* d_result, double factor, void* stream, int reps) {
+  try {
+    plan_autotune(lowdin::own_plan(*static_cast<Species*>(lint)), d_dens, d_result, &factor,
+                  static_cast<hipStream_t>(stream), std::max(reps, 1));
+    lowdin::g_last_error.clear();
+    return 0;
+  } catch (const std::exception& e) {
+    lowdin::set_last_error(std::string("autotune: ") + e.what());
+    return -1;
+  }
+}
+
+int lowdin_fock_targets(void* lint, int* targets) {
+  const lowdin::BuildPlan& p = lowdin::own_plan(*static_cast<Species*>(lint));
+  for (int k = 0; k < lowdin::kNumClassPairs; ++k) targets[k] = p.kind[k] == 2 ? 0 : p.target_wg[k];
+  return lowdin::kNumClassPairs;
 }
 
 int lowdin_fock_batch_targets(void* batch, int* targets) {

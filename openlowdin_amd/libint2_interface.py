@@ -77,6 +77,10 @@ def lib():
         L.libintinterface_compute_g12inter_direct.argtypes = [vp, vp, dp, dp, dp, dp, C.c_int]
         L.lowdin_fock_g12_device.argtypes = [vp, vp, vp, vp, C.c_double, dp, dp, C.c_int]
         L.lowdin_fock_g12_device.restype = C.c_int
+        L.lowdin_fock_autotune.argtypes = [vp, vp, vp, C.c_double, vp, C.c_int]
+        L.lowdin_fock_autotune.restype = C.c_int
+        L.lowdin_fock_targets.argtypes = [vp, ip]
+        L.lowdin_fock_targets.restype = C.c_int
         L.lowdin_fock_last_error.restype = C.c_char_p
         L.lowdin_fock_get_stats.argtypes = [vp, C.POINTER(Stats)]
         L.lowdin_fock_shellpair_list.argtypes = [vp, ip, ip, C.c_int]
@@ -373,6 +377,18 @@ class Libint2Interface:
         if rc != 0:
             _check("compute_2body_device")
             raise LowdinFockError("compute_2body_device failed")
+
+    def autotune(self, d_dens: int, d_result: int, factor: float, stream: int = 0, reps: int = 1):
+        """Per-class kernel shape of this species' build chosen by timing
+        (wavefront kernel or LDS strips at 512 / 2048 / 8192 workgroups)."""
+        if lib().lowdin_fock_autotune(self.this, d_dens, d_result, factor, stream or None, reps) != 0:
+            _check("autotune")
+            raise LowdinFockError("autotune failed")
+
+    def targets(self):
+        t = (C.c_int * 21)()
+        lib().lowdin_fock_targets(self.this, t)
+        return list(t)
 
     def time_launches(self, enable: bool = True):
         lib().lowdin_fock_time_launches(self.this, 1 if enable else 0)
