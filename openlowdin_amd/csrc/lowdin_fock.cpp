@@ -542,10 +542,10 @@ static void build_plan_tasks(BuildPlan& p, hipStream_t st) {
       const bool strip = strip_ok && (kd == 1 || (kd == 0 && total >= 192 * nbras));
       const size_t from = ts.tasks.size();
       // run length: at least enough kets to amortise zeroing and flushing the
-      // strips (which grow with n), at most 8192
+      // strips (which grow with n), at most 32768
       const long cmin = std::max(64L, std::min(256L, (long)((p.nmax / 2 + 63) / 64) * 64));
       long chunk = (total + p.target_wg[class_pair_id(cx, cy)] - 1) / p.target_wg[class_pair_id(cx, cy)];
-      chunk = std::max(cmin, std::min(8192L, ((chunk + 63) / 64) * 64));
+      chunk = std::max(cmin, std::min(32768L, ((chunk + 63) / 64) * 64));
       for (int m = 0; m < nm; ++m) {
         const PairSet& ps = p.mols[m]->sig;
         const int nx = ps.off[cx + 1] - ps.off[cx], ny = ps.off[cy + 1] - ps.off[cy];
