@@ -1304,13 +1304,20 @@ static std::vector<double> coeff_row(const double* coeff, int n, int p) {
 //   which 0: intra All, 1: intra IT, 2: coupling All, 3: coupling IT.
 static void run_store(Species& a, Species* b, int which, const double* dens, const double* coeff, int p,
                       double* result) {
+  // arguments are checked before the device is touched
+  const int mode = which & 1;
+  std::vector<double> row;
+  if (mode == 1) row = coeff_row(coeff, a.nbasis, p);
+  if (which >= 2 && !b) throw std::runtime_error("coupling variants need the other species");
+  if (which == 2 && a.species_id > b->species_id && a.nbasis != b->nbasis)
+    throw std::runtime_error("coupling_directAll: called from the higher species id with a different basis size "
+                             "(the reference layout indexes outside the n_a^2 n_b^2 result)");
   Context& c = ctx();
   std::lock_guard<std::mutex> lk(c.mu);
   const double t0 = now_ms();
   ensure_context();
   hipStream_t st = c.stream;
   const size_t na = a.nbasis, nb = b ? b->nbasis : 0;
-  const int mode = which & 1;
   size_t count = 0;
   switch (which) {
     case 0: count = na * na * na * na; break;
@@ -1328,9 +1335,8 @@ static void run_store(Species& a, Species* b, int which, const double* dens, con
   }
   const double* dC = nullptr;
   if (mode == 1) {
-    const std::vector<double> row = coeff_row(coeff, (int)na, p);
     a.d_cvec.upload(row.data(), na, st);
-    HIPCHK(hipStreamSynchronize(st));   // row is a stack-owned host buffer
+    HIPCHK(hipStreamSynchronize(st));   // row is a host buffer of this call
     dC = a.d_cvec.p;
   }
   lowdin_fock_stats stt{};

@@ -52,3 +52,36 @@ def test_generated_kernels_cover_all_classes():
     assert len(t) == 21
     for k in t:
         assert os.path.exists(os.path.join(REPO, "openlowdin_amd", "csrc", "gen", "k_%s.hip" % k))
+
+
+def test_store_and_g12_argument_errors_before_device_work():
+    """Argument errors of the transform, inter-species store and G12 entries
+    are found before the device is touched: the call records an error, a
+    transform's or build's result is NaN, a store's result is untouched."""
+    import ctypes
+    from openlowdin_amd import libint2_interface as LI
+    from openlowdin_amd import system as S
+    sp = S.water("6-31G**").species[0]
+    h = LI.Libint2Interface(sp, 1)
+    n = h.nbasis
+    with pytest.raises(LI.LowdinFockError, match="orbital index p out of range"):
+        h.compute_2body_intraspecies_direct_IT(np.eye(n), np.eye(n), n)
+    with pytest.raises(LI.LowdinFockError, match="orbital index p out of range"):
+        h.compute_2body_intraspecies_direct_IT(np.eye(n), np.eye(n), -1)
+    e, p = S.fhf_anion("6-31G").species
+    he, hp = LI.Libint2Interface(e, 1), LI.Libint2Interface(p, 2)
+    init = np.full(e.nbasis ** 2 * p.nbasis ** 2, 5.0)
+    with pytest.raises(LI.LowdinFockError, match="different basis size"):
+        hp.compute_2body_interspecies_direct_all(he, np.eye(p.nbasis), init)
+    # G12: more geminal terms than the kernels take, or a negative exponent
+    L = LI.lib()
+    G = np.zeros((n, n), order="F")
+    f = ctypes.c_double(-0.5)
+    for cs, gs in (([1.0] * 65, [0.5] * 65), ([1.0], [-1.0])):
+        c = np.array(cs)
+        g = np.array(gs)
+        D = np.asfortranarray(np.eye(n))
+        L.libintinterface_compute_g12_direct(h.this, LI._dptr(D), LI._dptr(G), ctypes.byref(f), LI._dptr(c),
+                                             LI._dptr(g), len(cs))
+        assert L.lowdin_fock_last_error().decode()
+        assert np.all(np.isnan(G))

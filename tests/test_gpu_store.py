@@ -148,6 +148,27 @@ def test_coupling_store_refuses_permuted_different_sizes():
         hp.compute_2body_interspecies_direct_all(he, np.eye(p.nbasis), init)
 
 
+def test_coupling_store_permuted_equal_sizes():
+    """From the higher species id with equal basis sizes (two species on one
+    basis, as alpha and beta electrons) the reference's strides fit: the
+    store is indexed with x of the lower species id."""
+    LI = _li()
+    sp = S.water("6-31G").species[0]
+    n = sp.nbasis
+    ha, hb = LI.Libint2Interface(sp, 1), LI.Libint2Interface(sp, 2)
+    b = O.Basis(sp)
+    _case("coupling_directAll permuted, equal sizes: water 6-31G (n=%d) as species 1 and 2" % n)
+    init = np.full(n ** 4, 1.5)
+    T = hb.compute_2body_interspecies_direct_all(ha, np.eye(n), init)
+    Tr = init.reshape(n, n, n, n).copy()
+    Tt = O.compute_coupling_directAll(b, b, 2, 1)
+    keep = Tt != 0.0
+    Tr[keep] = Tt[keep]
+    _store_close(T, Tr)
+    # one species, every pair: the tensor is the full ERI tensor, eight-fold symmetric where kept
+    assert np.max(np.abs(T - T.transpose(2, 3, 0, 1))[keep & keep.transpose(2, 3, 0, 1)]) < 1e-12
+
+
 def test_store_device_entry_matches_host_entry():
     LI = _li()
     from openlowdin_amd import hip as H
