@@ -298,6 +298,65 @@ def store_bench(H, LI, S, steps, D510, h510, eris510, cpu=True):
     return out
 
 
+def g12_bench(H, LI, S, steps, cpu=True):
+    """Gaussian-geminal builds with a synthetic 26-term geminal (exponents
+    10^(1 - k/8), k = 0..24, plus a constant term; alternating weights):
+      Fock build (factor -1) of the water tetramer 6-31G** (n=100), device
+        buffers;
+      coupling of the FHF- 6-311++G** electrons from the proton density;
+      CPU: the oracle's build on the water dimer beside the GPU's."""
+    g = [10.0 ** (1.0 - k / 8.0) for k in range(25)] + [0.0]
+    c = [(1.0 if k % 2 == 0 else -0.5) / (1.0 + k) for k in range(26)]
+    out = {"geminal_terms": len(g)}
+    sp = S.water_cluster(4, np.random.default_rng(2)).species[0]
+    n = sp.nbasis
+    h = LI.Libint2Interface(sp, 1)
+    h.init_2body_ints()
+    D = synthetic_density(n, sp.n_occupied, seed=3)
+    dD = H.DeviceArray.from_host(np.ascontiguousarray(D))
+    dG = H.DeviceArray((n * n,))
+    h.g12_device(dD.ptr, dG.ptr, c, g, factor=-1.0)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        h.g12_device(dD.ptr, dG.ptr, c, g, factor=-1.0)
+    dt = (time.perf_counter() - t0) / steps
+    st = h.stats()
+    out["fock_water4"] = {"n": n, "ms_per_build": round(dt * 1e3, 3), "kernel_ms": round(st["kernel_ms"], 3),
+                          "quartets": st["quartets"], "eris_per_s": round(st["eris"] / dt, 1)}
+    h.close()
+    f = S.fhf_anion("6-311++G**")
+    e, p = f.species
+    he, hp = LI.Libint2Interface(e, 1), LI.Libint2Interface(p, 2)
+    Dp = synthetic_density(p.nbasis, 1, seed=4) * 0.5
+    dDp = H.DeviceArray.from_host(np.ascontiguousarray(Dp))
+    dB = H.DeviceArray((e.nbasis * e.nbasis,))
+    he.g12_device(dDp.ptr, dB.ptr, c, g, other=hp)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        he.g12_device(dDp.ptr, dB.ptr, c, g, other=hp)
+    dt = (time.perf_counter() - t0) / steps
+    out["coupling_fhf"] = {"n_e": e.nbasis, "n_p": p.nbasis, "ms_per_build": round(dt * 1e3, 3)}
+    if cpu:
+        sys.path.insert(0, os.path.join(HERE, "oracle"))
+        import oracle as O
+        sp = S.water_cluster(2, np.random.default_rng(11)).species[0]
+        m = sp.nbasis
+        Dm = synthetic_density(m, sp.n_occupied, seed=5)
+        h = LI.Libint2Interface(sp, 1)
+        Gg = h.compute_g12_intraspecies_direct(Dm, -1.0, c, g)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            Gg = h.compute_g12_intraspecies_direct(Dm, -1.0, c, g)
+        tg = (time.perf_counter() - t0) / steps
+        t0 = time.perf_counter()
+        Gr = O.compute_g12_direct(O.Basis(sp), Dm, -1.0, c, g, nthreads=1)
+        tc = time.perf_counter() - t0
+        out["fock_water2_vs_cpu"] = {"n": m, "gpu_ms_host_buffers": round(tg * 1e3, 3), "cpu_ms": round(tc * 1e3, 1),
+                                     "cores": 1, "kind": "port", "max_abs_diff": float(np.max(np.abs(Gg - Gr)))}
+        h.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -314,6 +373,7 @@ def main():
     ap.add_argument("--serving-rates", default="0.5,1,2")
     ap.add_argument("--no-store", action="store_true", help="skip the integral store / transform runs")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default per-class kernel shapes")
+    ap.add_argument("--no-g12", action="store_true", help="skip the Gaussian-geminal builds")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 256 MiB device-to-device copy (profiler byte-counter calibration)")
     args = ap.parse_args()
@@ -461,6 +521,9 @@ def main():
         print("[bench] integral store / first-quarter transform...", file=sys.stderr, flush=True)
         out["store"] = store_bench(H, LI, S, max(2, min(args.steps, 5)), D, h, st["eris"],
                                    cpu=not args.no_cpu_baseline)
+    if not args.no_g12:
+        print("[bench] G12 builds...", file=sys.stderr, flush=True)
+        out["g12"] = g12_bench(H, LI, S, max(2, min(args.steps, 5)), cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle sample)...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(sp, D, args.cpu_stride, st["eris"])

@@ -59,8 +59,9 @@ __device__ __forceinline__ void g12_quartet(const double* bprim, int nbp, const 
       const double base = bp[8] * Kcd * pq * half_sqrt_pi;
       for (int k = 0; k < gem.n; ++k) {
         const double g = gem.g[k];
-        const double iD = 1.0 / (pq + g * spq);
-        const double pref = gem.c[k] * base * iD * sqrt(iD) * exp(-pq * g * R2 * iD);
+        const double r = rsqrt_nr(pq + g * spq);   // D^(-1/2)
+        const double iD = r * r;
+        const double pref = gem.c[k] * base * iD * r * exp(-pq * g * R2 * iD);
         const double gqD = g * q * iD, gpD = g * p * iD;
         G::template vrr<double>(bp[5] - gqD * PQx, bp[6] - gqD * PQy, bp[7] - gqD * PQz, QCx + gpD * PQx,
                                 QCy + gpD * PQy, QCz + gpD * PQz, 0.5 * (q + g) * iD, 0.5 * g * iD,

@@ -1071,10 +1071,12 @@ static void enqueue_store(Species& s, const double* dD, const double* dC, double
   const bool screen = s.is_electron;
   if (screen) launch_dnorm(dD, n, s.d_shells.p, ns, s.d_Dn1.ensure((size_t)ns * ns), st);
   if (mode == 1) launch_fill_zero(dOut, (size_t)n * n * n, st);
-  hipStream_t hs = c.side[0];
   HIPCHK(hipEventRecord(c.fork, st));
-  HIPCHK(hipStreamWaitEvent(hs, c.fork, 0));
+  for (int q = 0; q < kSideStreams; ++q) HIPCHK(hipStreamWaitEvent(c.side[q], c.fork, 0));
+  size_t nlight = 0;
   for (const TaskSet::Launch& L : ts.launches) {
+    // scratch-heavy classes on side stream 0, the others round robin over 1..3
+    hipStream_t hs = heavy_launch(L) ? c.side[0] : c.side[1 + nlight++ % (kSideStreams - 1)];
     StoreArgs g{};
     g.tasks = ts.d_tasks.p + L.off;
     g.ntasks = (int)L.count;
@@ -1097,8 +1099,7 @@ static void enqueue_store(Species& s, const double* dD, const double* dC, double
     count_tasks(ts, L, s.sig.pairs.data() + s.sig.off[L.cbra], s.sig.pairs.data() + s.sig.off[L.cket], stt);
   }
   HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(c.join[0], hs));
-  HIPCHK(hipStreamWaitEvent(st, c.join[0], 0));
+  join_side_streams(st);
   if (mode == 1) launch_mirror_upper(dOut, n, n, st);
   HIPCHK(hipGetLastError());
 }
@@ -1125,10 +1126,11 @@ static void enqueue_coupling_store(Species& a, Species& b, const double* dC, dou
   TaskSet* ts = coupling_taskset(a, b, st);
   const size_t na = a.nbasis, nb = b.nbasis;
   if (mode == 1) launch_fill_zero(dOut, na * nb * nb, st);
-  hipStream_t hs = c.side[0];
   HIPCHK(hipEventRecord(c.fork, st));
-  HIPCHK(hipStreamWaitEvent(hs, c.fork, 0));
+  for (int q = 0; q < kSideStreams; ++q) HIPCHK(hipStreamWaitEvent(c.side[q], c.fork, 0));
+  size_t nlight = 0;
   for (const TaskSet::Launch& L : ts->launches) {
+    hipStream_t hs = heavy_launch(L) ? c.side[0] : c.side[1 + nlight++ % (kSideStreams - 1)];
     const bool a_bra = L.bra_is_a != 0;
     Species& sb = a_bra ? a : b;
     Species& sk = a_bra ? b : a;
@@ -1156,8 +1158,7 @@ static void enqueue_coupling_store(Species& a, Species& b, const double* dC, dou
     count_tasks(*ts, L, sb.all.pairs.data() + sb.all.off[L.cbra], sk.all.pairs.data() + sk.all.off[L.cket], stt);
   }
   HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(c.join[0], hs));
-  HIPCHK(hipStreamWaitEvent(st, c.join[0], 0));
+  join_side_streams(st);
   if (mode == 1) launch_mirror_upper(dOut, (int)na, (int)nb, st);
   HIPCHK(hipGetLastError());
 }
@@ -1188,10 +1189,12 @@ static void enqueue_g12(Species& s, const double* dD, double* dG, double factor,
   const int n = s.nbasis;
   double* graw = s.d_Graw1.ensure(std::max<size_t>((size_t)n * n, 1));
   launch_fill_zero(graw, (size_t)n * n, st);
-  hipStream_t hs = c.side[0];
   HIPCHK(hipEventRecord(c.fork, st));
-  HIPCHK(hipStreamWaitEvent(hs, c.fork, 0));
+  for (int q = 0; q < kSideStreams; ++q) HIPCHK(hipStreamWaitEvent(c.side[q], c.fork, 0));
+  size_t nlight = 0;
   for (const TaskSet::Launch& L : ts.launches) {
+    // scratch-heavy classes on side stream 0, the others round robin over 1..3
+    hipStream_t hs = heavy_launch(L) ? c.side[0] : c.side[1 + nlight++ % (kSideStreams - 1)];
     G12Args g{};
     g.tasks = ts.d_tasks.p + L.off;
     g.ntasks = (int)L.count;
@@ -1209,8 +1212,7 @@ static void enqueue_g12(Species& s, const double* dD, double* dG, double factor,
     count_tasks(ts, L, s.sig.pairs.data() + s.sig.off[L.cbra], s.sig.pairs.data() + s.sig.off[L.cket], stt);
   }
   HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(c.join[0], hs));
-  HIPCHK(hipStreamWaitEvent(st, c.join[0], 0));
+  join_side_streams(st);
   launch_symmetrize(graw, n, 1.0, dG, st);
   HIPCHK(hipGetLastError());
 }
@@ -1225,10 +1227,11 @@ static void enqueue_g12_coupling(Species& a, Species& b, const double* dDb, doub
   ensure_all(b, st);
   TaskSet* ts = coupling_taskset(a, b, st);
   launch_fill_zero(dRa, (size_t)a.nbasis * a.nbasis, st);
-  hipStream_t hs = c.side[0];
   HIPCHK(hipEventRecord(c.fork, st));
-  HIPCHK(hipStreamWaitEvent(hs, c.fork, 0));
+  for (int q = 0; q < kSideStreams; ++q) HIPCHK(hipStreamWaitEvent(c.side[q], c.fork, 0));
+  size_t nlight = 0;
   for (const TaskSet::Launch& L : ts->launches) {
+    hipStream_t hs = heavy_launch(L) ? c.side[0] : c.side[1 + nlight++ % (kSideStreams - 1)];
     const bool a_bra = L.bra_is_a != 0;
     Species& sb = a_bra ? a : b;
     Species& sk = a_bra ? b : a;
@@ -1258,8 +1261,7 @@ static void enqueue_g12_coupling(Species& a, Species& b, const double* dDb, doub
     count_tasks(*ts, L, sb.all.pairs.data() + sb.all.off[L.cbra], sk.all.pairs.data() + sk.all.off[L.cket], stt);
   }
   HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(c.join[0], hs));
-  HIPCHK(hipStreamWaitEvent(st, c.join[0], 0));
+  join_side_streams(st);
 }
 
 // Host-pointer G12 build (b null: intra-species on a with a's density;
