@@ -263,16 +263,18 @@ int lowdin_fock_batch_launch_profile(void* batch, int cap, int* classes, float* 
                                      uint64_t* prim_quartets);
 
 /* Autotuner: for each class launch, the kernel shape is chosen among the
- * wavefront kernel (one bra x 64 kets per wave) and the LDS-strip kernel
- * cut into about {512, 2048, 8192} workgroups, by timing `reps` serialised
- * builds per candidate on the given inputs.  Results are unchanged by the
- * choice up to FP64 summation order. */
+ * wavefront kernel (one bra x 64 kets per wave), the LDS-strip kernel cut
+ * into about {512, 2048, 8192} workgroups of 256 threads, and the LDS-strip
+ * kernel with one-wavefront workgroups (about 512), by timing `reps`
+ * serialised builds per candidate on the given inputs.  Results are
+ * unchanged by the choice up to FP64 summation order. */
 int lowdin_fock_batch_autotune(void* batch, const double* d_dens, double* d_result, const double* factors,
                                void* stream, int reps);
 
 /* The 21 per-class choices (class pair id cx*(cx+1)/2+cy over the pair
  * classes ss, ps, pp, ds, dp, dd): 0 for the wavefront kernel, else the
- * strip kernel's workgroup target.  Returns 21. */
+ * strip kernel's workgroup target, negative for one-wavefront workgroups.
+ * Returns 21. */
 int lowdin_fock_batch_targets(void* batch, int* targets);
 
 /* The same autotuner and choices for one species' own build (the plan behind

@@ -486,7 +486,8 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
 
 // ------------------------------------------------ J/K kernel, LDS strips
 // A workgroup owns one bra pair (a b) and streams up to t.j1 - t.j0 ket
-// pairs, one per thread per round.  Every J/K update of a quartet except
+// pairs, one per thread per round (256 threads, or one wavefront of 64 for
+// the short ket runs of small molecules).  Every J/K update of a quartet except
 // J_cd lands in row a or row b of G (J_ab; K_ac, K_ad in row a; K_bc, K_bd
 // in row b), so those are accumulated with LDS atomics in two row strips
 // (NA + NB) x n held for the whole ket stream and flushed to HBM once, with
@@ -799,7 +800,8 @@ template <int LA, int LB, int LC, int LD>
 static void launch_jk_strip_t(const JKArgs& a, hipStream_t s) {
   if (a.ntasks <= 0) return;
   const size_t lds = (size_t)(ncart(LA) + ncart(LB)) * a.lds_n * sizeof(double);
-  hipLaunchKernelGGL((jk_strip_kernel<LA, LB, LC, LD>), dim3(a.ntasks), dim3(256), lds, s, a);
+  hipLaunchKernelGGL((jk_strip_kernel<LA, LB, LC, LD>), dim3(a.ntasks), dim3(a.block > 0 ? a.block : 256), lds, s,
+                     a);
 }
 
 template <int LA, int LB, int LC, int LD>

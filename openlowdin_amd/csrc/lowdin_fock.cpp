@@ -170,7 +170,7 @@ struct TaskSet {
     int cbra, cket;   // pair classes of bra / ket lists
     int bra_is_a;     // coupling: bra list belongs to the first species
     size_t off, count;
-    int strip;        // intra: one workgroup per task with LDS row strips
+    int strip;        // intra: one workgroup per task with LDS row strips (1: 256 threads, 2: 64)
   };
   std::vector<Launch> launches;
   std::vector<size_t> order;   // launch order: estimated cost, largest first
@@ -243,7 +243,8 @@ struct BuildPlan {
   DevBuf<int> d_ranges;                 // per class launch: first task, task count
   uint64_t layout = 0;                  // bumps whenever tasks or buffers move
   int target_wg[kNumClassPairs];        // workgroups a strip launch aims at (autotuned)
-  int kind[kNumClassPairs];             // 0: choose by ket run length, 1: LDS strips, 2: wavefront tasks
+  int kind[kNumClassPairs];             // 0: choose by ket run length, 1: LDS strips, 2: wavefront tasks,
+                                        // 3: LDS strips with one-wavefront workgroups
   lowdin_fock_stats stats{};
   std::vector<float> launch_ms;         // per class launch of the last build
   std::vector<unsigned long long> launch_cnt;
@@ -539,7 +540,7 @@ static void build_plan_tasks(BuildPlan& p, hipStream_t st) {
       const int kd = p.kind[class_pair_id(cx, cy)];
       const bool strip_ok = (size_t)(ncart_h(la) + ncart_h(lb)) * p.nmax * sizeof(double) <= kStripLdsMax &&
                             maxbra <= kMaxStripPrims;
-      const bool strip = strip_ok && (kd == 1 || (kd == 0 && total >= 192 * nbras));
+      const bool strip = strip_ok && (kd == 1 || kd == 3 || (kd == 0 && total >= 192 * nbras));
       const size_t from = ts.tasks.size();
       // run length: at least enough kets to amortise zeroing and flushing the
       // strips (which grow with n), at most 32768
@@ -566,7 +567,7 @@ static void build_plan_tasks(BuildPlan& p, hipStream_t st) {
       }
       std::stable_sort(ts.tasks.begin() + from, ts.tasks.end(),
                        [&](const Task& x, const Task& y) { return task_cost(x, cx, cy) > task_cost(y, cx, cy); });
-      ts.launches.push_back({la, lb, lc, ld, cx, cy, 1, from, ts.tasks.size() - from, strip ? 1 : 0});
+      ts.launches.push_back({la, lb, lc, ld, cx, cy, 1, from, ts.tasks.size() - from, strip ? (kd == 3 ? 2 : 1) : 0});
       // estimated cost of the launch: primitive quartets x class weight
       const double w = 1.0 + (la + lb + lc + ld) * (la + lb + lc + ld);
       double c = 0.0;
@@ -744,6 +745,7 @@ static void enqueue_classes(BuildPlan& p, hipStream_t st, bool timed, int which,
     a.cket = L.cket;
     a.same_class = L.cbra == L.cket ? 1 : 0;
     a.lds_n = p.nmax;
+    a.block = L.strip == 2 ? 64 : 256;
     a.eps = 2.220446049250313080847e-16;  // DBL_EPSILON, Libint2Iface.h:269-271
     if (timed) HIPCHK(hipEventRecord(p.launch_ev[2 * k], ls));
     const bool ok = L.strip ? dispatch_jk_strip(L.la, L.lb, L.lc, L.ld, a, ls) : dispatch_jk(L.la, L.lb, L.lc, L.ld, a, ls);
@@ -1930,9 +1932,9 @@ static int plan_autotune(lowdin::BuildPlan& p, const double* d_dens, double* d_r
   std::lock_guard<std::mutex> lk(c.mu);
   lowdin::ensure_context();
   hipStream_t st = stream ? stream : c.stream;
-  // candidate 0 is the wavefront kernel, the rest LDS strips aiming at that
-  // many workgroups per launch
-  static const int cand[] = {0, 512, 2048, 8192};
+  // candidates: the wavefront kernel, LDS strips aiming at that many
+  // workgroups per launch, and LDS strips with one-wavefront workgroups
+  static const int cand[][2] = {{2, 2048}, {1, 512}, {1, 2048}, {1, 8192}, {3, 512}};   // {kind, workgroups}
   const bool tl = p.time_launches, se = p.serialize, ug = p.use_graph;
   p.time_launches = true;
   p.serialize = true;
@@ -1945,10 +1947,11 @@ static int plan_autotune(lowdin::BuildPlan& p, const double* d_dens, double* d_r
     pick_kind[k] = p.kind[k];
   }
   try {
-    for (int wg : cand) {
+    for (const auto& cd : cand) {
+      const int kind = cd[0], wg = cd[1];
       for (int k = 0; k < lowdin::kNumClassPairs; ++k) {
-        p.kind[k] = wg == 0 ? 2 : 1;
-        p.target_wg[k] = wg == 0 ? 2048 : wg;
+        p.kind[k] = kind;
+        p.target_wg[k] = wg;
       }
       p.valid = false;
       std::vector<double> ms;
@@ -1964,11 +1967,11 @@ static int plan_autotune(lowdin::BuildPlan& p, const double* d_dens, double* d_r
       for (size_t k = 0; k < p.ts.launches.size(); ++k) {
         const lowdin::TaskSet::Launch& L = p.ts.launches[k];
         const int id = lowdin::class_pair_id(L.cbra, L.cket);
-        if (wg != 0 && !L.strip) continue;   // strips impossible for this class: candidate ran as waves
+        if (kind != 2 && !L.strip) continue;   // strips impossible for this class: candidate ran as waves
         if (ms[k] < 0.97 * best[id]) {
           best[id] = ms[k];
-          pick[id] = wg == 0 ? 2048 : wg;
-          pick_kind[id] = wg == 0 ? 2 : 1;
+          pick[id] = wg;
+          pick_kind[id] = kind;
         }
       }
     }
@@ -2022,13 +2025,15 @@ int lowdin_fock_autotune(void* lint, const double* d_dens, double* d_result, dou
 
 int lowdin_fock_targets(void* lint, int* targets) {
   const lowdin::BuildPlan& p = lowdin::own_plan(*static_cast<Species*>(lint));
-  for (int k = 0; k < lowdin::kNumClassPairs; ++k) targets[k] = p.kind[k] == 2 ? 0 : p.target_wg[k];
+  for (int k = 0; k < lowdin::kNumClassPairs; ++k)
+    targets[k] = p.kind[k] == 2 ? 0 : (p.kind[k] == 3 ? -p.target_wg[k] : p.target_wg[k]);
   return lowdin::kNumClassPairs;
 }
 
 int lowdin_fock_batch_targets(void* batch, int* targets) {
   const lowdin::BuildPlan& p = static_cast<lowdin_fock_batch_t*>(batch)->plan;
-  for (int k = 0; k < lowdin::kNumClassPairs; ++k) targets[k] = p.kind[k] == 2 ? 0 : p.target_wg[k];
+  for (int k = 0; k < lowdin::kNumClassPairs; ++k)
+    targets[k] = p.kind[k] == 2 ? 0 : (p.kind[k] == 3 ? -p.target_wg[k] : p.target_wg[k]);
   return lowdin::kNumClassPairs;
 }
 

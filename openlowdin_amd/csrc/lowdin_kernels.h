@@ -39,6 +39,7 @@ struct JKArgs {
   int cbra, cket;          // pair classes of the bra and ket lists
   int same_class;          // bra and ket lists are the same class list
   int lds_n;               // largest n of the launch (LDS strip sizing)
+  int block;               // strip kernel: threads per workgroup (0: 256)
   double eps;
 };
 
