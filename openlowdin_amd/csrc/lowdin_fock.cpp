@@ -540,7 +540,11 @@ static void build_plan_tasks(BuildPlan& p, hipStream_t st) {
       const int kd = p.kind[class_pair_id(cx, cy)];
       const bool strip_ok = (size_t)(ncart_h(la) + ncart_h(lb)) * p.nmax * sizeof(double) <= kStripLdsMax &&
                             maxbra <= kMaxStripPrims;
-      const bool strip = strip_ok && (kd == 1 || kd == 3 || (kd == 0 && total >= 192 * nbras));
+      // untuned: strips wherever they fit, of 256 threads when a bra has on
+      // average >= 192 kets, else of one wavefront (short ket runs of small
+      // molecules would leave most of a 256-thread workgroup idle)
+      const bool strip = strip_ok && (kd == 1 || kd == 3 || kd == 0);
+      const bool wave64 = kd == 3 || (kd == 0 && total < 192 * nbras);
       const size_t from = ts.tasks.size();
       // run length: at least enough kets to amortise zeroing and flushing the
       // strips (which grow with n), at most 32768
@@ -567,7 +571,7 @@ static void build_plan_tasks(BuildPlan& p, hipStream_t st) {
       }
       std::stable_sort(ts.tasks.begin() + from, ts.tasks.end(),
                        [&](const Task& x, const Task& y) { return task_cost(x, cx, cy) > task_cost(y, cx, cy); });
-      ts.launches.push_back({la, lb, lc, ld, cx, cy, 1, from, ts.tasks.size() - from, strip ? (kd == 3 ? 2 : 1) : 0});
+      ts.launches.push_back({la, lb, lc, ld, cx, cy, 1, from, ts.tasks.size() - from, strip ? (wave64 ? 2 : 1) : 0});
       // estimated cost of the launch: primitive quartets x class weight
       const double w = 1.0 + (la + lb + lc + ld) * (la + lb + lc + ld);
       double c = 0.0;
