@@ -32,7 +32,10 @@ program fock_host_driver
   real(8), allocatable :: orig(:,:), ex(:), co(:), tmpe(:), tmpc(:)
   real(8), allocatable, target :: G(:,:), aux(:,:), coupling(:,:)
   real(8) :: factor
-  integer :: i, i2
+  integer :: i, i2, ng, p
+  logical :: have
+  real(8), allocatable :: gexp(:), gcoef(:)
+  real(8), allocatable, target :: A3(:,:,:)
 
   call get_command_argument(1, dir)
   open(newunit=u, file=trim(dir)//"/job.txt", status="old", action="read")
@@ -91,6 +94,42 @@ program fock_host_driver
      call write_matrix(trim(dir)//"/coupling_"//itoa(ids(k))//".bin", coupling)
      deallocate(coupling)
   end do
+
+  ! optional: Gaussian-geminal builds (g12.txt: term count, exponents,
+  ! coefficients) and first-quarter transforms (it.txt: orbital row p, the
+  ! density used as the coefficient matrix)
+  inquire(file=trim(dir)//"/g12.txt", exist=have)
+  if (have) then
+     open(newunit=u, file=trim(dir)//"/g12.txt", status="old", action="read")
+     read(u, *) ng
+     allocate(gexp(ng), gcoef(ng))
+     read(u, *) gexp
+     read(u, *) gcoef
+     close(u)
+     do k = 1, ns
+        factor = kappa(k) / eta(k)
+        call Libint2Interface_computeG12Intraspecies_direct(dens(k)%v, G, factor, gcoef, gexp, objs(k))
+        call write_matrix(trim(dir)//"/g12_"//itoa(ids(k))//".bin", G)
+        do j = 1, ns
+           if (j == k) cycle
+           call Libint2Interface_computeG12Interspecies_direct(dens(j)%v, aux, gcoef, gexp, objs(k), objs(j))
+           call write_matrix(trim(dir)//"/g12c_"//itoa(ids(k))//"_"//itoa(ids(j))//".bin", aux)
+        end do
+     end do
+  end if
+  inquire(file=trim(dir)//"/it.txt", exist=have)
+  if (have) then
+     open(newunit=u, file=trim(dir)//"/it.txt", status="old", action="read")
+     read(u, *) p
+     close(u)
+     do k = 1, ns
+        call Libint2Interface_compute2BodyIntraspecies_direct_IT(dens(k)%v, dens(k)%v, A3, p, objs(k))
+        open(newunit=u, file=trim(dir)//"/it_"//itoa(ids(k))//".bin", access="stream", form="unformatted", &
+             status="replace", action="write")
+        write(u) A3
+        close(u)
+     end do
+  end if
 
   do k = 1, ns
      call Libint2Interface_destructor(objs(k))

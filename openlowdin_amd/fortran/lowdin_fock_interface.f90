@@ -11,6 +11,10 @@
 !!   DirectIntegralManager_getDirectIntraRepulsionMatrix
 !!   DirectIntegralManager_getDirectInterRepulsionMatrix
 !!                                           (DirectIntegralManager.f90:66-179)
+!!   Libint2Interface_compute2Body{Intra,Inter}species_direct_IT / _all,
+!!   Libint2Interface_computeG12{Intra,Inter}species_direct
+!!                                           (Libint2Interface.f90:551-707,
+!!                                            935-1179)
 !! The reference constructor reads shells from its MolecularSystem; here the
 !! shells arrive as flat arrays (angular momentum, primitive count, origin,
 !! exponents, raw coefficients), in the same particle-then-contraction order.
@@ -79,6 +83,47 @@ module LowdinFockInterface_
        type(c_ptr), value :: result
      end subroutine c_LibintInterface_computeCouplingDirect
 
+     subroutine c_LibintInterface_compute2BodyDirectIT(this, density, coefficients, result, p) &
+          bind(C, name="LibintInterface_compute_2body_directIT")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: this, density, coefficients, result
+       integer(c_int) :: p
+     end subroutine c_LibintInterface_compute2BodyDirectIT
+
+     subroutine c_LibintInterface_computeCouplingDirectIT(this, othis, density, coefficients, result, p) &
+          bind(C, name="LibintInterface_compute_coupling_directIT")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: this, othis, density, coefficients, result
+       integer(c_int) :: p
+     end subroutine c_LibintInterface_computeCouplingDirectIT
+
+     subroutine c_LibintInterface_compute2BodyDirectAll(this, density, result) &
+          bind(C, name="LibintInterface_compute_2body_directAll")
+       import :: c_ptr
+       type(c_ptr), value :: this, density, result
+     end subroutine c_LibintInterface_compute2BodyDirectAll
+
+     subroutine c_LibintInterface_computeCouplingDirectAll(this, othis, density, result) &
+          bind(C, name="LibintInterface_compute_coupling_directAll")
+       import :: c_ptr
+       type(c_ptr), value :: this, othis, density, result
+     end subroutine c_LibintInterface_computeCouplingDirectAll
+
+     subroutine c_LibintInterface_computeG12Direct(this, density, result, factor, coefficients, exponents, pot_size) &
+          bind(C, name="libintinterface_compute_g12_direct")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: this, density, result, coefficients, exponents
+       integer(c_int), value :: pot_size
+       real(c_double) :: factor
+     end subroutine c_LibintInterface_computeG12Direct
+
+     subroutine c_LibintInterface_computeG12InterDirect(this, othis, density, result, coefficients, exponents, &
+          pot_size) bind(C, name="libintinterface_compute_g12inter_direct")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: this, othis, density, result, coefficients, exponents
+       integer(c_int), value :: pot_size
+     end subroutine c_LibintInterface_computeG12InterDirect
+
      function c_lowdin_fock_last_error() result(msg) bind(C, name="lowdin_fock_last_error")
        import :: c_ptr
        type(c_ptr) :: msg
@@ -94,6 +139,9 @@ module LowdinFockInterface_
   public :: Libint2Interface_constructor, Libint2Interface_destructor, &
        Libint2Interface_compute2BodyIntraspecies_direct, Libint2Interface_compute2BodyInterspecies_direct, &
        DirectIntegralManager_getDirectIntraRepulsionMatrix, DirectIntegralManager_getDirectInterRepulsionMatrix, &
+       Libint2Interface_compute2BodyIntraspecies_direct_IT, Libint2Interface_compute2BodyInterspecies_direct_IT, &
+       Libint2Interface_compute2BodyIntraspecies_direct_all, Libint2Interface_compute2BodyInterspecies_direct_all, &
+       Libint2Interface_computeG12Intraspecies_direct, Libint2Interface_computeG12Interspecies_direct, &
        LowdinFock_checkError
 
 contains
@@ -188,6 +236,93 @@ contains
     call c_LibintInterface_computeCouplingDirect(this%this, other%this, c_loc(density(1,1)), c_loc(coupling(1,1)))
     call LowdinFock_checkError("compute_coupling_direct")
   end subroutine Libint2Interface_compute2BodyInterspecies_direct
+
+  !> Allocates matrixA (n, n, n) and computes the first-quarter transform on
+  !! coefficient row p: matrixA(l, k, j) = sum_i C(p, i) (i j|k l), the C
+  !! image of result[j n^2 + k n + l] (Libint2Interface.f90:551-579).
+  subroutine Libint2Interface_compute2BodyIntraspecies_direct_IT(density, coefficients, matrixA, p, this)
+    real(8), target, intent(in) :: density(:,:), coefficients(:,:)
+    real(8), allocatable, target, intent(inout) :: matrixA(:,:,:)
+    integer, intent(in) :: p
+    type(Libint2Interface), intent(inout) :: this
+    integer(c_int) :: pp
+    if (allocated(matrixA)) deallocate(matrixA)
+    allocate(matrixA(this%nbasis, this%nbasis, this%nbasis))
+    matrixA = 0.0_8
+    pp = int(p, c_int)
+    call c_LibintInterface_compute2BodyDirectIT(this%this, c_loc(density(1,1)), c_loc(coefficients(1,1)), &
+         c_loc(matrixA(1,1,1)), pp)
+    call LowdinFock_checkError("compute_2body_directIT")
+  end subroutine Libint2Interface_compute2BodyIntraspecies_direct_IT
+
+  !> coupling (n_other, n_other, n_this) (Libint2Interface.f90:677-707).
+  subroutine Libint2Interface_compute2BodyInterspecies_direct_IT(density, coefficients, coupling, p, this, other)
+    real(8), target, intent(in) :: density(:,:), coefficients(:,:)
+    real(8), allocatable, target, intent(inout) :: coupling(:,:,:)
+    integer, intent(in) :: p
+    type(Libint2Interface), intent(inout) :: this, other
+    integer(c_int) :: pp
+    if (allocated(coupling)) deallocate(coupling)
+    allocate(coupling(other%nbasis, other%nbasis, this%nbasis))
+    pp = int(p, c_int)
+    call c_LibintInterface_computeCouplingDirectIT(this%this, other%this, c_loc(density(1,1)), &
+         c_loc(coefficients(1,1)), c_loc(coupling(1,1,1)), pp)
+    call LowdinFock_checkError("compute_coupling_directIT")
+  end subroutine Libint2Interface_compute2BodyInterspecies_direct_IT
+
+  !> ints (n, n, n, n), caller-allocated: the Coulomb branch of
+  !! Libint2Interface.f90:1065-1115.
+  subroutine Libint2Interface_compute2BodyIntraspecies_direct_all(density, ints, this)
+    real(8), target, intent(in) :: density(:,:)
+    real(8), target, intent(inout) :: ints(:,:,:,:)
+    type(Libint2Interface), intent(inout) :: this
+    call c_LibintInterface_compute2BodyDirectAll(this%this, c_loc(density(1,1)), c_loc(ints(1,1,1,1)))
+    call LowdinFock_checkError("compute_2body_directAll")
+  end subroutine Libint2Interface_compute2BodyIntraspecies_direct_all
+
+  !> The Coulomb branch of Libint2Interface.f90:1117-1179.
+  subroutine Libint2Interface_compute2BodyInterspecies_direct_all(density, ints, this, other)
+    real(8), target, intent(in) :: density(:,:)
+    real(8), target, intent(inout) :: ints(:,:,:,:)
+    type(Libint2Interface), intent(inout) :: this, other
+    call c_LibintInterface_computeCouplingDirectAll(this%this, other%this, c_loc(density(1,1)), c_loc(ints(1,1,1,1)))
+    call LowdinFock_checkError("compute_coupling_directAll")
+  end subroutine Libint2Interface_compute2BodyInterspecies_direct_all
+
+  !> Allocates twoBody, rebuilds the pair list, computes J + factor K with
+  !! the geminal sum_k coefficients(k) exp(-exponents(k) r12^2)
+  !! (Libint2Interface.f90:935-992; the potential's components are passed
+  !! in instead of looked up in the InterPotential table).
+  subroutine Libint2Interface_computeG12Intraspecies_direct(density, twoBody, factor, coefficients, exponents, this)
+    real(8), target, intent(in) :: density(:,:)
+    real(8), allocatable, target, intent(inout) :: twoBody(:,:)
+    real(8), intent(in) :: factor
+    real(8), target, intent(in) :: coefficients(:), exponents(:)
+    type(Libint2Interface), intent(inout) :: this
+    real(c_double) :: f
+    if (allocated(twoBody)) deallocate(twoBody)
+    allocate(twoBody(this%nbasis, this%nbasis))
+    f = factor
+    call c_LibintInterface_init2BodyInts(this%this)
+    call LowdinFock_checkError("init_2body_ints")
+    call c_LibintInterface_computeG12Direct(this%this, c_loc(density(1,1)), c_loc(twoBody(1,1)), f, &
+         c_loc(coefficients(1)), c_loc(exponents(1)), int(size(coefficients), c_int))
+    call LowdinFock_checkError("compute_g12_direct")
+  end subroutine Libint2Interface_computeG12Intraspecies_direct
+
+  !> Allocates coupling (n_this x n_this): the geminal coupling on `this`
+  !! from the other species' density (Libint2Interface.f90:995-1061).
+  subroutine Libint2Interface_computeG12Interspecies_direct(density, coupling, coefficients, exponents, this, other)
+    real(8), target, intent(in) :: density(:,:)
+    real(8), allocatable, target, intent(inout) :: coupling(:,:)
+    real(8), target, intent(in) :: coefficients(:), exponents(:)
+    type(Libint2Interface), intent(inout) :: this, other
+    if (allocated(coupling)) deallocate(coupling)
+    allocate(coupling(this%nbasis, this%nbasis))
+    call c_LibintInterface_computeG12InterDirect(this%this, other%this, c_loc(density(1,1)), c_loc(coupling(1,1)), &
+         c_loc(coefficients(1)), c_loc(exponents(1)), int(size(coefficients), c_int))
+    call LowdinFock_checkError("compute_g12inter_direct")
+  end subroutine Libint2Interface_computeG12Interspecies_direct
 
   !> DirectIntegralManager.f90:66-130: copy D, require the LIBINT scheme,
   !! call the species' handle.

@@ -74,3 +74,45 @@ def test_fortran_driver_fhf_anion(tmp_path):
     up = np.triu(c12)
     expect = -(up + np.triu(up, 1).T)
     assert np.max(np.abs(cpl1 - expect)) < 1e-14
+
+
+@pytest.mark.gpu
+def test_fortran_driver_g12_and_first_quarter(tmp_path):
+    """The Fortran layer's G12 and first-quarter-transform wrappers (the
+    bind(C) names of Libint2Interface.f90) against the oracle: the geminal
+    Fock and coupling builds of FHF- and the transform on orbital row 3."""
+    system = S.fhf_anion("6-31G")
+    e, p = system.species
+    rng = np.random.default_rng(29)
+    Ce = rng.normal(size=(e.nbasis, 10)) / np.sqrt(e.nbasis)
+    De = 2.0 * Ce @ Ce.T
+    cp = rng.normal(size=p.nbasis) / np.sqrt(p.nbasis)
+    Dp = np.outer(cp, cp)
+    _write_job(str(tmp_path), system, [De, Dp])
+    gexp = [4.0, 1.2, 0.3, 0.0]
+    gcoef = [0.8, -0.35, 0.6, 0.05]
+    with open(os.path.join(str(tmp_path), "g12.txt"), "w") as fh:
+        fh.write("%d\n%s\n%s\n" % (len(gexp), " ".join("%.17g" % x for x in gexp),
+                                   " ".join("%.17g" % x for x in gcoef)))
+    with open(os.path.join(str(tmp_path), "it.txt"), "w") as fh:
+        fh.write("3\n")
+    print("\n[case] fortran driver: FHF- 6-31G + proton, G12 (4 terms) and first quarter p=3", flush=True)
+    r = subprocess.run([DRIVER, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    be, bp = O.Basis(e), O.Basis(p)
+    g1 = _read(str(tmp_path), "g12_1.bin", e.nbasis)
+    assert np.max(np.abs(g1 - O.compute_g12_direct(be, De, -0.5, gcoef, gexp))) < 1e-10
+    g2 = _read(str(tmp_path), "g12_2.bin", p.nbasis)
+    assert np.max(np.abs(g2 - O.compute_g12_direct(bp, Dp, -1.0, gcoef, gexp))) < 1e-10
+    c12 = _read(str(tmp_path), "g12c_1_2.bin", e.nbasis)
+    ref12 = O.compute_g12inter_direct(be, bp, 1, 2, Dp, gcoef, gexp)
+    assert np.max(np.abs(c12.ravel(order="F") - ref12.ravel())) < 1e-10
+    c21 = _read(str(tmp_path), "g12c_2_1.bin", p.nbasis)
+    ref21 = O.compute_g12inter_direct(bp, be, 2, 1, De, gcoef, gexp)
+    assert np.max(np.abs(c21.ravel(order="F") - ref21.ravel())) < 1e-10
+    # first quarter: the Fortran array (n, n, n) holds the C image j n^2 + k n + l
+    n = e.nbasis
+    it = np.fromfile(os.path.join(str(tmp_path), "it_1.bin"), dtype=np.float64).reshape(n, n, n)
+    ref = O.compute_2body_directIT(be, De, De, 3)
+    assert np.max(np.abs(it - ref)) < 1e-10
