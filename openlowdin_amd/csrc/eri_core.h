@@ -484,6 +484,27 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
     }
 }
 
+// Waves per SIMD the strip kernel of a class is compiled for (the register
+// budget: 512 / waves).  Default 1 (no limit).  The classes listed measured
+// faster with the tighter budget on C20H42 / cc-pVDZ: (ps|ss) at 5 waves
+// (85 VGPRs, no spill; 4 waves unrestricted), (ds|ds) and (dd|ss) at 2
+// (a few hundred bytes of spill against occupancy 1).  Others measured
+// slower: (ps|ps), (pp|ss), (ds|ss) at 5 waves by 6-22 %, and every class
+// with more than 256 registers at 2 waves (e.g. (dp|pp) 1.6 -> 3.6 ms).
+template <int LA, int LB, int LC, int LD>
+struct StripWaves {
+  static constexpr int value = 1;
+};
+#define LOWDIN_STRIP_WAVES(a, b, c, d, w) \
+  template <>                              \
+  struct StripWaves<a, b, c, d> {          \
+    static constexpr int value = w;        \
+  };
+LOWDIN_STRIP_WAVES(1, 0, 0, 0, 5)
+LOWDIN_STRIP_WAVES(2, 0, 2, 0, 2)
+LOWDIN_STRIP_WAVES(2, 2, 0, 0, 2)
+#undef LOWDIN_STRIP_WAVES
+
 // ------------------------------------------------ J/K kernel, LDS strips
 // A workgroup owns one bra pair (a b) and streams up to t.j1 - t.j0 ket
 // pairs, one per thread per round (256 threads, or one wavefront of 64 for
@@ -499,7 +520,7 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
 // through global-address-space views its integrals came out wrong on the
 // GPU (cause not found; the FP64 path is unaffected).
 template <int LA, int LB, int LC, int LD>
-__global__ __launch_bounds__(256) void jk_strip_kernel(JKArgs a) {
+__global__ __launch_bounds__(256, (StripWaves<LA, LB, LC, LD>::value)) void jk_strip_kernel(JKArgs a) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr bool kF32 = Fp32Allowed<LA, LB, LC, LD>::value;
   extern __shared__ double strip[];  // rows of a (NA x n), then rows of b (NB x n)
