@@ -489,8 +489,9 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
 // faster with the tighter budget on C20H42 / cc-pVDZ: (ps|ss) at 5 waves
 // (85 VGPRs, no spill; 4 waves unrestricted), (ds|ds) and (dd|ss) at 2
 // (a few hundred bytes of spill against occupancy 1).  Others measured
-// slower: (ps|ps), (pp|ss), (ds|ss) at 5 waves by 6-22 %, and every class
-// with more than 256 registers at 2 waves (e.g. (dp|pp) 1.6 -> 3.6 ms).
+// slower: (ps|ps), (pp|ss), (ds|ss) at 5 waves by 6-22 %, (ps|ss) at 6,
+// (ds|ps), (dp|ss), (pp|ps) at 3, and the other classes with more than
+// 256 registers at 2 waves (e.g. (dp|pp) 1.6 -> 3.6 ms).
 template <int LA, int LB, int LC, int LD>
 struct StripWaves {
   static constexpr int value = 1;
