@@ -130,6 +130,29 @@ void libintinterface_compute_g12inter_direct(void* lint, void* olint, double* de
                                              const double* coefficients, const double* exponents,
                                              const int pot_size);
 
+/* ------------------------------------------------ link-level stubs
+ * The rest of the reference's exported C ABI (Libint2Iface.h:381-445,
+ * bound by Libint2Interface.f90:165-373): disk storage, the alpha-beta
+ * build and the G12 store variants are outside this library's scope.  They
+ * are exported so that openLOWDIN's Fortran links unchanged; a call records
+ * an error (lowdin_fock_last_error), prints it and fills its result array,
+ * where it has one, with NaN (alphabeta: 1 value; g12_directAll: n^4;
+ * g12inter_directAll: n_lint^2 n_olint^2).  The disk variants write no
+ * file. */
+void LibintInterface_compute_2body_disk(void* lint, const char* filename, double* dens);
+void LibintInterface_compute_alphabeta_direct(void* lint, void* olint, double* dens, double* otherdens,
+                                              double* result);
+void LibintInterface_compute_coupling_disk(void* lint, void* olint, const char* filename);
+void libintinterface_compute_g12_disk(void* lint, const char* filename, const double* coefficients,
+                                      const double* exponents, const int pot_size);
+void libintinterface_compute_g12inter_disk(void* lint, void* olint, const char* filename,
+                                           const double* coefficients, const double* exponents, const int pot_size);
+void libintinterface_compute_g12_directAll(void* lint, double* dens, double* result, const double* coefficients,
+                                           const double* exponents, const int pot_size);
+void libintinterface_compute_g12inter_directAll(void* lint, void* olint, double* dens, double* result,
+                                                const double* coefficients, const double* exponents,
+                                                const int pot_size);
+
 /* ------------------------------------------------ extensions (no reference
  * counterpart; used by this package's host code, tests and benchmarks) */
 

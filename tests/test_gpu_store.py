@@ -3,8 +3,8 @@ integral-transformation variants of the two-particle path) through the C
 ABI, against the oracle.
 
 Bars: store elements within 1e-12 absolute of the oracle, except that an
-integral whose raw value lies within rounding of the 1e-10 drop may be kept
-on one side and dropped on the other (at most a handful, each below 1e-9);
+integral whose raw value lies within rounding (1e-9 relative) of the 1e-10
+drop may be kept on one side and dropped on the other;
 elements the store does not write keep the caller's values; transforms
 within 1e-11 absolute.  Every test prints its case before the GPU call.
 """
@@ -31,13 +31,29 @@ def _case(msg):
     print("\n[case] " + msg, flush=True)
 
 
-def _store_close(got, ref):
+def _store_close(got, ref, init, nprod):
+    """Store parity.  An element one side kept and the other left at the
+    caller's value ``init`` is a drop-decision mismatch; it is allowed only
+    where the kept integral's raw value (element / product of the four
+    functions' norma, ``nprod``) lies within 1e-9 relative of the 1e-10
+    drop threshold, i.e. within rounding of the cut.  Every other element
+    agrees to 1e-12."""
+    got, ref = got.reshape(-1), ref.reshape(-1)
+    nprod = np.broadcast_to(nprod, got.shape).reshape(-1)
+    mism = (got == init) != (ref == init)
+    kept = np.where(got == init, ref, got)
+    raw = np.abs(kept[mism] / nprod[mism])
+    near = np.abs(raw - 1e-10) <= 1e-9 * 1e-10
     d = np.abs(got - ref)
-    bad = d > 1e-12
-    print("   max|dA| = %.3e, elements over 1e-12: %d" % (d.max(), int(bad.sum())))
-    assert int(bad.sum()) <= 64
-    if bad.any():
-        assert d[bad].max() < 1e-9
+    d[mism] = 0.0
+    print("   max|dA| = %.3e over agreeing elements; drop-decision mismatches %d (near the cut %d)"
+          % (d.max(), int(mism.sum()), int(near.sum())))
+    assert near.all()
+    assert d.max() <= 1e-12
+
+
+def _nprod4(n1, n2, n3, n4):
+    return np.einsum("i,j,k,l->ijkl", n1, n2, n3, n4)
 
 
 INTRA = {
@@ -59,8 +75,10 @@ def test_store_all_matches_oracle(name):
     h.init_2body_ints()
     init = np.full(n ** 4, 3.0)
     A = h.compute_2body_intraspecies_direct_all(D, init)
-    Ar = O.compute_2body_directAll(O.Basis(sp), D, A=init)
-    _store_close(A, Ar)       # also checks that unwritten elements kept the caller's 3.0
+    b = O.Basis(sp)
+    Ar = O.compute_2body_directAll(b, D, A=init)
+    nb = O.norma(b)
+    _store_close(A, Ar, 3.0, _nprod4(nb, nb, nb, nb))  # unwritten elements kept the caller's 3.0
     st = h.stats()
     assert st["quartets"] > 0 and st["kernel_ms"] > 0
 
@@ -122,7 +140,8 @@ def test_coupling_store_and_transform(ebasis):
     Tt = O.compute_coupling_directAll(be, bp, 1, 2)
     keep = Tt != 0.0
     Tr[keep] = Tt[keep]
-    _store_close(T, Tr)
+    ne, npn = O.norma(be), O.norma(bp)
+    _store_close(T, Tr, -2.0, _nprod4(ne, ne, npn, npn).reshape(T.shape))
     for this, other, hthis, hother, it, io in ((be, bp, he, hp, 1, 2), (bp, be, hp, he, 2, 1)):
         n = this.n
         Cm = rng.normal(size=(n, n))
@@ -164,7 +183,8 @@ def test_coupling_store_permuted_equal_sizes():
     Tt = O.compute_coupling_directAll(b, b, 2, 1)
     keep = Tt != 0.0
     Tr[keep] = Tt[keep]
-    _store_close(T, Tr)
+    nb = O.norma(b)
+    _store_close(T, Tr, 1.5, _nprod4(nb, nb, nb, nb))
     # one species, every pair: the tensor is the full ERI tensor, eight-fold symmetric where kept
     assert np.max(np.abs(T - T.transpose(2, 3, 0, 1))[keep & keep.transpose(2, 3, 0, 1)]) < 1e-12
 
