@@ -287,8 +287,10 @@ int lowdin_fock_batch_launch_profile(void* batch, int cap, int* classes, float* 
 
 /* Autotuner: for each class launch, the kernel shape is chosen among the
  * wavefront kernel (one bra x 64 kets per wave), the LDS-strip kernel cut
- * into about {512, 2048, 8192} workgroups of 256 threads, and the LDS-strip
- * kernel with one-wavefront workgroups (about 512), by timing `reps`
+ * into about {512, 2048, 8192} workgroups of 256 threads, the LDS-strip
+ * kernel with one-wavefront workgroups (about 512) and, for total angular
+ * momentum >= 5, the wave-cooperative kernel (one quartet per wavefront,
+ * with or without row strips, about {512, 2048} workgroups), by timing `reps`
  * serialised builds per candidate on the given inputs.  Results are
  * unchanged by the choice up to FP64 summation order. */
 int lowdin_fock_batch_autotune(void* batch, const double* d_dens, double* d_result, const double* factors,
@@ -296,8 +298,10 @@ int lowdin_fock_batch_autotune(void* batch, const double* d_dens, double* d_resu
 
 /* The 21 per-class choices (class pair id cx*(cx+1)/2+cy over the pair
  * classes ss, ps, pp, ds, dp, dd): 0 for the wavefront kernel, else the
- * strip kernel's workgroup target, negative for one-wavefront workgroups.
- * Returns 21. */
+ * strip kernel's workgroup target, negative for one-wavefront workgroups,
+ * 1000000 + target for the wave-cooperative kernel with LDS row strips and
+ * 2000000 + target for the wave-cooperative kernel without strips (classes
+ * of total angular momentum >= 5).  Returns 21. */
 int lowdin_fock_batch_targets(void* batch, int* targets);
 
 /* The same autotuner and choices for one species' own build (the plan behind
@@ -323,6 +327,8 @@ typedef struct {
   double charge;   /* particle charge q */
   double factor;   /* exchange factor kappa / eta of the direct build */
   int build_two;   /* 0: G = 0 (one particle, WaveFunction.f90:867) */
+  int ghost;       /* 1: SCF_GHOST_SPECIES -- its coupling is left out of the other species'
+                      Fock matrices (WaveFunction.f90:1502-1503), kept in the energy */
 } lowdin_scf_species;
 
 /* lints: species handles (shells added); e_pc: point-charge repulsion;

@@ -27,6 +27,7 @@ WANTED = {
     "NAKAI-3-S": ["H_1", "H-A_1", "H-B_1"],
     "NAKAI-3-SP": ["H_1"],
     "DZSPNB": ["H_1"],
+    "DZSNB": ["H_1"],
     "GRIBAKIN-10S": ["E+"],
 }
 

@@ -1,0 +1,345 @@
+// Wave-cooperative J/K kernel for the high angular momentum classes.
+//
+// The per-lane kernels of eri_core.h hold a whole quartet's recursion in
+// one lane's registers.  From total L = 6 up ((dp|dp), (dd|pp), (dd|ds),
+// (dd|dp), (dd|dd)) that state no longer fits: the kernels spill kilobytes
+// per lane to scratch, and the few quartets of those classes (3959 (dd|dd)
+// of C20H42 / cc-pVDZ: 62 wavefronts for 1024 SIMDs) leave the GPU idle.
+// Here one wavefront evaluates one quartet and the 64 lanes share each
+// recursion level:
+//   * the Obara-Saika VRR ([e0|f0]^(m) from the scaled Boys values) and the
+//     Head-Gordon-Pople HRR ((ab|cd) from [e0|f0]) run as precomputed
+//     rounds of at most 64 independent operations, one per lane, whose
+//     operands live in the wavefront's LDS region (gen_eri.py, CoopGen,
+//     writes each class's round table and LDS layout);
+//   * the contracted [e0|f0] accumulate in LDS over the primitive quartets;
+//   * the J/K contraction of Libint2Iface.cpp:562-586 splits the six
+//     updates over the lanes (one output element per lane, its sum over
+//     the other two indices in LDS).
+// A workgroup owns one bra pair and a run of kets (the strip kernel's
+// task), its wavefronts take kets in turn.  With LDS room for the bra's two
+// row strips ((NA + NB) x n doubles), J_ab and the four exchange blocks
+// accumulate there and are flushed once per workgroup; J_cd is one global
+// atomic per element; without strips every update is a global atomic.
+// Within one wavefront LDS operations complete in issue order, so a round
+// reads what the previous round wrote without a barrier.
+#pragma once
+#include "eri_core.h"
+
+namespace lowdin {
+
+template <int LA, int LB, int LC, int LD>
+struct CoopProgram;
+
+// Fixed slots of a wavefront's LDS region (doubles).
+constexpr int kCoopZero = 18;   // 18 and 19: always 0.0 (m and m+1 operands of absent terms)
+constexpr int kCoopAcc = 22;    // first contracted accumulator (20: target of idle lanes)
+
+__device__ __forceinline__ double lds_at(const double* R, unsigned byteoff) {
+  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(R) + byteoff);
+}
+__device__ __forceinline__ double& lds_ref(double* R, unsigned byteoff) {
+  return *reinterpret_cast<double*>(reinterpret_cast<char*>(R) + byteoff);
+}
+
+// Doubles of one wavefront's region: the program region, then the six
+// normalised density blocks (ab, cd, ac, bd, ad, bc), rounded up to even.
+template <int LA, int LB, int LC, int LD>
+__host__ __device__ constexpr int coop_wave_doubles() {
+  constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  return (CoopProgram<LA, LB, LC, LD>::kRegion + NA * NB + NC * ND + NA * NC + NB * ND + NA * ND + NB * NC + 1) & ~1;
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(256, 2) void jk_coop_kernel(JKArgs a) {
+  using Pg = CoopProgram<LA, LB, LC, LD>;
+  constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  constexpr int L = LA + LB + LC + LD;
+  constexpr int NAB = NA * NB, NCD = NC * ND;
+  constexpr int NACC = EriClass<LA, LB, LC, LD>::nacc;
+  constexpr int kWave = coop_wave_doubles<LA, LB, LC, LD>();
+  constexpr int oDab = Pg::kRegion, oDcd = oDab + NAB, oDac = oDcd + NCD, oDbd = oDac + NA * NC,
+                oDad = oDbd + NB * ND, oDbc = oDad + NA * ND;
+  static_assert(NAB <= 64, "one J_ab element per lane");
+  extern __shared__ double lds[];
+  __shared__ unsigned int wg_q, wg_p;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int nw = blockDim.x >> 6;
+  const Task t = a.tasks[blockIdx.x];
+  const MolDesc& M = a.mols[t.flags];
+  const gcptr<PairInfo> pairs = (gcptr<PairInfo>)M.pairs;
+  const gcptr<double> prim = (gcptr<double>)M.prim;
+  const gcptr<double> D = (gcptr<double>)M.D;
+  const gcptr<double> nrm = (gcptr<double>)M.norma;
+  const gptr<double> G = (gptr<double>)M.Graw;
+  const PairInfo P = load_pair(pairs + M.off[a.cbra] + t.bra);
+  const int n = M.n, ns = M.nshell;
+  const int sn = a.lds_n;                 // strip width (0: no strips)
+  double* sa = lds;
+  double* sb = lds + NA * sn;
+  double* R = lds + (NA + NB) * sn + w * kWave;
+  for (int k = tid; k < (NA + NB) * sn; k += blockDim.x) lds[k] = 0.0;
+  if (tid == 0) {
+    wg_q = 0;
+    wg_p = 0;
+  }
+  if (lane < 2) R[kCoopZero + lane] = 0.0;
+  // normalised bra density block; lane k < NAB owns J_ab element k
+  for (int k = lane; k < NAB; k += 64) {
+    const int ia = k / NB, ib = k % NB;
+    R[oDab + k] = D[(size_t)(P.bfa + ia) * n + P.bfb + ib] * nrm[P.bfa + ia] * nrm[P.bfb + ib];
+  }
+  double jab = 0.0;
+  unsigned int myq = 0, myp = 0;
+  const double sbra = (P.sa != P.sb ? 2.0 : 1.0);
+  const double hf = M.half_factor;
+  const gcptr<double> bprim = prim + (size_t)P.poff * kPrimPairStride;
+  const uint2* pg = Pg::prog() + lane;   // this lane's column of the round table
+  __syncthreads();
+  for (int j = t.j0 + w; j < t.j1; j += nw) {
+    const PairInfo K = load_pair(pairs + M.off[a.cket] + j);
+    if (M.screen) {
+      const double dn = quartet_dnorm((gcptr<double>)M.Dn, ns, P, K);
+      if (dn * P.Q * K.Q < a.eps) continue;
+    }
+    myq += 1;
+    myp += (unsigned int)(P.nprim * K.nprim);
+    for (int k = lane; k < NACC; k += 64) R[kCoopAcc + k] = 0.0;
+    const gcptr<double> kprim = prim + (size_t)K.poff * kPrimPairStride;
+    for (int kp = 0; kp < K.nprim; ++kp) {
+      const gcptr<double> kq = kprim + kp * kPrimPairStride;
+      const double q = kq[0], o2q = kq[1];
+      const double Qx = kq[2], Qy = kq[3], Qz = kq[4];
+      const double Kcd = kq[8];
+      for (int ip = 0; ip < P.nprim; ++ip) {
+        const gcptr<double> bp = bprim + ip * kPrimPairStride;
+        const double p = bp[0], o2p = bp[1];
+        const double PQx = bp[2] - Qx, PQy = bp[3] - Qy, PQz = bp[4] - Qz;
+        const double rs = rsqrt_nr(p + q);
+        const double inv = rs * rs;
+        const double rop = q * inv, roq = p * inv;
+        const double T = p * rop * (PQx * PQx + PQy * PQy + PQz * PQz);
+        const double pref = bp[8] * Kcd * rs;
+        double F[L + 1];
+        boys_f64<L>(T, a.boys, F);
+        if (lane == 0) {
+          R[0] = bp[5];
+          R[1] = -rop * PQx;
+          R[2] = bp[6];
+          R[3] = -rop * PQy;
+          R[4] = bp[7];
+          R[5] = -rop * PQz;
+          R[6] = kq[5];
+          R[7] = roq * PQx;
+          R[8] = kq[6];
+          R[9] = roq * PQy;
+          R[10] = kq[7];
+          R[11] = roq * PQz;
+        }
+#pragma unroll
+        for (int m = 0; m <= L; ++m)
+          if (lane == m) R[Pg::kBase + m] = F[m] * pref;
+        const double o2pq = 0.5 * inv;
+        // VRR rounds (runtime loop, next round's descriptor loaded one round ahead)
+        uint2 nx = pg[0];
+        #pragma unroll 1
+        for (int r = 0; r < Pg::kVrrRounds; ++r) {
+          const uint2 d = nx;
+          nx = pg[(r + 1) * 64];
+          const bool ket = ((r < 32 ? (Pg::kVrrKind >> r) : (Pg::kVrrKindHi >> (r - 32))) & 1u) != 0;
+          const unsigned s0 = d.x & 0xfff8u, k1 = d.x & 7u;
+          const unsigned s2 = (d.x >> 16) & 0xfff8u, k2 = (d.x >> 16) & 7u;
+          const unsigned dst = (d.y >> 16) & 0xfff8u, i = (d.y >> 16) & 3u;
+          const double x0 = lds_at(R, s0), x1 = lds_at(R, s0 + 8);
+          const double x2 = lds_at(R, s2), x3 = lds_at(R, s2 + 8);
+          double v;
+          if (ket) {
+            const double x4 = lds_at(R, d.y & 0xfff8u);
+            v = R[6 + 2 * i] * x0 + R[7 + 2 * i] * x1 + ((double)k1 * o2q) * (x2 - roq * x3) +
+                ((double)k2 * o2pq) * x4;
+          } else {
+            v = R[2 * i] * x0 + R[1 + 2 * i] * x1 + ((double)k1 * o2p) * (x2 - rop * x3);
+          }
+          lds_ref(R, dst) = v;
+        }
+        // accumulate the [e0|f0]^(0) targets
+        #pragma unroll 1
+        for (int r = 0; r < Pg::kAccRounds; ++r) {
+          const uint2 d = nx;
+          nx = pg[(Pg::kVrrRounds + r + 1) * 64];
+          lds_ref(R, d.y) += lds_at(R, d.x);
+        }
+      }
+    }
+    // HRR rounds
+    if (lane == 0) {
+      R[12] = P.ABx;
+      R[13] = P.ABy;
+      R[14] = P.ABz;
+      R[15] = K.ABx;
+      R[16] = K.ABy;
+      R[17] = K.ABz;
+    }
+    {
+      const uint2* hp = pg + (Pg::kVrrRounds + Pg::kAccRounds) * 64;
+      uint2 nx = hp[0];
+      #pragma unroll 1
+      for (int r = 0; r < Pg::kHrrRounds; ++r) {
+        const uint2 d = nx;
+        nx = hp[(r + 1) * 64];
+        const bool ket = ((r < 64 ? (Pg::kHrrKind0 >> r) : (Pg::kHrrKind1 >> (r - 64))) & 1ull) != 0;
+        const unsigned dst = d.y & 0xfff8u, i = d.y & 3u;
+        lds_ref(R, dst) = lds_at(R, d.x & 0xffffu) + R[(ket ? 15 : 12) + i] * lds_at(R, d.x >> 16);
+      }
+    }
+    // J/K contraction of the (ab|cd) block (Libint2Iface.cpp:562-586)
+    const double* out = R + Pg::kOut;
+    const double deg = sbra * (K.sa != K.sb ? 2.0 : 1.0) * ((a.same_class && j == t.bra) ? 1.0 : 2.0);
+    for (int k = lane; k < NCD + NA * NC + NB * ND + NA * ND + NB * NC; k += 64) {
+      int r0, c0, o;
+      if (k < NCD) {
+        r0 = K.bfa + k / ND; c0 = K.bfb + k % ND; o = oDcd + k;
+      } else if (k < NCD + NA * NC) {
+        const int u = k - NCD;
+        r0 = P.bfa + u / NC; c0 = K.bfa + u % NC; o = oDac + u;
+      } else if (k < NCD + NA * NC + NB * ND) {
+        const int u = k - NCD - NA * NC;
+        r0 = P.bfb + u / ND; c0 = K.bfb + u % ND; o = oDbd + u;
+      } else if (k < NCD + NA * NC + NB * ND + NA * ND) {
+        const int u = k - NCD - NA * NC - NB * ND;
+        r0 = P.bfa + u / ND; c0 = K.bfb + u % ND; o = oDad + u;
+      } else {
+        const int u = k - NCD - NA * NC - NB * ND - NA * ND;
+        r0 = P.bfb + u / NC; c0 = K.bfa + u % NC; o = oDbc + u;
+      }
+      R[o] = D[(size_t)r0 * n + c0] * nrm[r0] * nrm[c0];
+    }
+    if (lane < NAB) {
+      double s = 0.0;
+#pragma unroll 6
+      for (int cd = 0; cd < NCD; ++cd) s += out[lane * NCD + cd] * R[oDcd + cd];
+      jab += deg * s;
+    }
+    if (lane < NCD) {
+      double s = 0.0;
+#pragma unroll 6
+      for (int ab = 0; ab < NAB; ++ab) s += out[ab * NCD + lane] * R[oDab + ab];
+      const int fc = K.bfa + lane / ND, fd = K.bfb + lane % ND;
+      global_add(G + (size_t)fc * n + fd, deg * s * nrm[fc] * nrm[fd]);
+    }
+    const double sk = deg * hf;
+    if (lane < NA * NC) {  // K_ac = sum_bd (ab|cd) D_bd
+      const int ia = lane / NC, ic = lane % NC;
+      double s = 0.0;
+#pragma unroll 6
+      for (int ib = 0; ib < NB; ++ib)
+#pragma unroll 6
+        for (int id = 0; id < ND; ++id) s += out[((ia * NB + ib) * NC + ic) * ND + id] * R[oDbd + ib * ND + id];
+      const int fc = K.bfa + ic;
+      if (sn) atomicAdd(&sa[ia * sn + fc], sk * s * nrm[fc]);
+      else global_add(G + (size_t)(P.bfa + ia) * n + fc, sk * s * nrm[fc] * nrm[P.bfa + ia]);
+    }
+    if (lane < NB * ND) {  // K_bd = sum_ac (ab|cd) D_ac
+      const int ib = lane / ND, id = lane % ND;
+      double s = 0.0;
+#pragma unroll 6
+      for (int ia = 0; ia < NA; ++ia)
+#pragma unroll 6
+        for (int ic = 0; ic < NC; ++ic) s += out[((ia * NB + ib) * NC + ic) * ND + id] * R[oDac + ia * NC + ic];
+      const int fd = K.bfb + id;
+      if (sn) atomicAdd(&sb[ib * sn + fd], sk * s * nrm[fd]);
+      else global_add(G + (size_t)(P.bfb + ib) * n + fd, sk * s * nrm[fd] * nrm[P.bfb + ib]);
+    }
+    if (lane < NA * ND) {  // K_ad = sum_bc (ab|cd) D_bc
+      const int ia = lane / ND, id = lane % ND;
+      double s = 0.0;
+#pragma unroll 6
+      for (int ib = 0; ib < NB; ++ib)
+#pragma unroll 6
+        for (int ic = 0; ic < NC; ++ic) s += out[((ia * NB + ib) * NC + ic) * ND + id] * R[oDbc + ib * NC + ic];
+      const int fd = K.bfb + id;
+      if (sn) atomicAdd(&sa[ia * sn + fd], sk * s * nrm[fd]);
+      else global_add(G + (size_t)(P.bfa + ia) * n + fd, sk * s * nrm[fd] * nrm[P.bfa + ia]);
+    }
+    if (lane < NB * NC) {  // K_bc = sum_ad (ab|cd) D_ad
+      const int ib = lane / NC, ic = lane % NC;
+      double s = 0.0;
+#pragma unroll 6
+      for (int ia = 0; ia < NA; ++ia)
+#pragma unroll 6
+        for (int id = 0; id < ND; ++id) s += out[((ia * NB + ib) * NC + ic) * ND + id] * R[oDad + ia * ND + id];
+      const int fc = K.bfa + ic;
+      if (sn) atomicAdd(&sb[ib * sn + fc], sk * s * nrm[fc]);
+      else global_add(G + (size_t)(P.bfb + ib) * n + fc, sk * s * nrm[fc] * nrm[P.bfb + ib]);
+    }
+  }
+  // J_ab of this wavefront (lane-owned elements)
+  if (lane < NAB && jab != 0.0) {
+    const int ia = lane / NB, ib = lane % NB;
+    const double v = jab * nrm[P.bfb + ib];
+    if (sn) atomicAdd(&sa[ia * sn + P.bfb + ib], v);
+    else global_add(G + (size_t)(P.bfa + ia) * n + P.bfb + ib, v * nrm[P.bfa + ia]);
+  }
+  if (a.counts && lane == 0) {
+    atomicAdd(&wg_q, myq);
+    atomicAdd(&wg_p, myp);
+  }
+  __syncthreads();
+  if (sn) {
+    for (int r = 0; r < NA; ++r) {
+      const double nr = nrm[P.bfa + r];
+      for (int c = tid; c < n; c += blockDim.x) {
+        const double v = sa[r * sn + c];
+        if (v != 0.0) global_add(G + (size_t)(P.bfa + r) * n + c, v * nr);
+      }
+    }
+    for (int r = 0; r < NB; ++r) {
+      const double nr = nrm[P.bfb + r];
+      for (int c = tid; c < n; c += blockDim.x) {
+        const double v = sb[r * sn + c];
+        if (v != 0.0) global_add(G + (size_t)(P.bfb + r) * n + c, v * nr);
+      }
+    }
+  }
+  if (a.counts && tid == 0) {
+    a.counts[3 * blockIdx.x] = wg_q;
+    a.counts[3 * blockIdx.x + 1] = wg_p;
+    a.counts[3 * blockIdx.x + 2] = 0;
+  }
+}
+
+// Dynamic LDS a cooperative workgroup may use (gfx950: 160 KB per workgroup,
+// less the kernel's static counters).
+constexpr size_t kCoopLdsMax = 163840 - 256;
+
+template <int LA, int LB, int LC, int LD>
+static void coop_attr_t() {
+  (void)hipFuncSetAttribute((const void*)jk_coop_kernel<LA, LB, LC, LD>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)kCoopLdsMax);
+}
+
+// One workgroup per task.  a.lds_n: row-strip width wanted (0: none);
+// a.block: threads wanted (64, 128, 192 or 256).  Fewer wavefronts are used
+// when strips and regions would not fit in LDS, no strips when even one
+// wavefront would not fit beside them.
+template <int LA, int LB, int LC, int LD>
+static bool launch_jk_coop_t(JKArgs a, hipStream_t s) {
+  if (a.ntasks <= 0) return true;
+  constexpr size_t wave = (size_t)coop_wave_doubles<LA, LB, LC, LD>() * sizeof(double);
+  size_t strip = (size_t)(ncart(LA) + ncart(LB)) * (size_t)a.lds_n * sizeof(double);
+  int W = (a.block > 0 ? a.block : 256) / 64;
+  W = W < 1 ? 1 : (W > 4 ? 4 : W);
+  while (W > 1 && strip + W * wave > kCoopLdsMax) --W;
+  if (strip + W * wave > kCoopLdsMax) {
+    a.lds_n = 0;
+    strip = 0;
+    W = (int)(kCoopLdsMax / wave);
+    W = W > 4 ? 4 : W;
+  }
+  if (W < 1) return false;
+  hipLaunchKernelGGL((jk_coop_kernel<LA, LB, LC, LD>), dim3(a.ntasks), dim3(64 * W), strip + W * wave, s, a);
+  return true;
+}
+
+}  // namespace lowdin

@@ -271,19 +271,386 @@ class HRRGen:
         return "\n".join([sig, unused] + self.lines + out + ["}"])
 
 
+
+# ------------------------------------------------------------------------
+# Wave-cooperative programs for the high-L classes.
+#
+# One wavefront evaluates one quartet; its intermediates live in LDS and the
+# 64 lanes share every recursion level.  The VRR and HRR above become
+# "rounds" of at most 64 independent operations (one per lane), grouped by
+# (dependency level, recursion kind) so that the kind, and with it the
+# recursion's coefficient vectors, is known at compile time; each lane reads
+# its operation's LDS operands from a per-class descriptor table.
+#
+# Per-wave LDS region (doubles):
+#   [0, 18)        coefficient vectors: (PA_i, WP_i) at 2i, (QC_i, WQ_i) at
+#                  6 + 2i, AB_i at 12 + i, CD_i at 15 + i
+#   18, 19         ZERO (always 0.0: operands of absent terms; a pair, since
+#                  a term reads the m and m+1 slots)
+#   20             TRASH (target of idle lanes)
+#   [22, 22+NACC)  contracted [e0|f0] accumulators
+#   [.., +NOUT)    (ab|cd) block in the HRR's output order (aliases the
+#                  accumulators when the class has no HRR)
+#   [.., +W)       work slots: VRR values of one primitive quartet, later
+#                  the HRR intermediates (slots reused by live range)
+# Descriptor words (uint2, byte offsets into the region, multiples of 8 so
+# that the low 3 bits carry small fields):
+#   VRR  x = s0 | k1 | (s2 | k2) << 16      y = s4 | (dst | i) << 16
+#        kind 0: V = PA_i V[s0] + WP_i V[s0+1] + k1 o2p (V[s2] - rop V[s2+1])
+#        kind 1: V = QC_i V[s0] + WQ_i V[s0+1] + k1 o2q (V[s2] - roq V[s2+1])
+#                    + k2 o2pq V[s4]
+#   ACC  x = src, y = dst                      acc[dst] += V[src]
+#   HRR  x = x | (y << 16), y = dst | i        V = V[x] + C_i V[y], C = AB (kind 0) or CD (kind 1)
+COOP_P, COOP_ZERO, COOP_TRASH, COOP_ACC = 0, 18, 20, 22
+
+
+class CoopGen:
+    def __init__(self, la, lb, lc, ld):
+        self.cls = (la, lb, lc, ld)
+        self.L = la + lb + lc + ld
+        self.es = [c for l in range(la, la + lb + 1) for c in comps(l)]
+        self.fs = [c for l in range(lc, lc + ld + 1) for c in comps(l)]
+        self.nacc = len(self.es) * len(self.fs)
+        self.nout = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld)
+
+    # ---- VRR dependency graph (the recursion VRRGen emits)
+    def _vrr_graph(self):
+        ops = {}     # key -> (kind, i, k1, k2, s0key, s2key, s4key)
+
+        def visit(key):
+            if key in ops:
+                return
+            e, f, m = key
+            if sum(e) == 0 and sum(f) == 0:
+                ops[key] = None
+                return
+            if sum(f) > 0:
+                i = pick(f)
+                f1 = add(f, i, -1)
+                s0 = (e, f1, m)
+                s2 = s4 = None
+                k1 = k2 = 0
+                if f1[i] > 0:
+                    s2, k1 = (e, add(f1, i, -1), m), f1[i]
+                if e[i] > 0:
+                    s4, k2 = (add(e, i, -1), f1, m + 1), e[i]
+                ops[key] = (1, i, k1, k2, s0, s2, s4)
+            else:
+                i = pick(e)
+                e1 = add(e, i, -1)
+                s0 = (e1, f, m)
+                s2 = None
+                k1 = 0
+                if e1[i] > 0:
+                    s2, k1 = (add(e1, i, -1), f, m), e1[i]
+                ops[key] = (0, i, k1, 0, s0, s2, None)
+            _, _, _, _, s0, s2, s4 = ops[key]
+            for s in (s0, s2):
+                if s is not None:
+                    visit(s)
+                    visit((s[0], s[1], s[2] + 1))
+            if s4 is not None:
+                visit(s4)
+
+        for e in self.es:
+            for f in self.fs:
+                visit((e, f, 0))
+        return ops
+
+    @staticmethod
+    def _alloc(runs, nrounds):
+        """Linear scan over rounds.  runs: name -> [size, def_round, last_use];
+        a run is placed before its defining round and released after its last
+        use.  Returns (offsets, peak)."""
+        free = [(0, 1 << 30)]
+        off = {}
+        peak = 0
+        order = sorted(runs, key=lambda r: (runs[r][1], -runs[r][0]))
+        by_def = {}
+        for r in order:
+            by_def.setdefault(runs[r][1], []).append(r)
+        by_end = {}
+        for r in runs:
+            by_end.setdefault(runs[r][2], []).append(r)
+        for rnd in range(-1, nrounds + 1):
+            for r in by_def.get(rnd, []):
+                size = runs[r][0]
+                for k, (a, n) in enumerate(free):
+                    if n >= size:
+                        off[r] = a
+                        free[k] = (a + size, n - size)
+                        peak = max(peak, a + size)
+                        break
+            for r in by_end.get(rnd, []):
+                a, size = off[r], runs[r][0]
+                free.append((a, size))
+                free.sort()
+                merged = []
+                for x in free:
+                    if merged and merged[-1][0] + merged[-1][1] == x[0]:
+                        merged[-1] = (merged[-1][0], merged[-1][1] + x[1])
+                    else:
+                        merged.append(x)
+                free = merged
+        return off, peak
+
+    def build(self):
+        L = self.L
+        ops = self._vrr_graph()
+        level = {}
+
+        def lev(key):
+            if key in level:
+                return level[key]
+            op = ops[key]
+            if op is None:
+                level[key] = 0
+                return 0
+            _, _, _, _, s0, s2, s4 = op
+            srcs = [s0, (s0[0], s0[1], s0[2] + 1)]
+            if s2 is not None:
+                srcs += [s2, (s2[0], s2[1], s2[2] + 1)]
+            if s4 is not None:
+                srcs.append(s4)
+            level[key] = 1 + max(lev(s) for s in srcs)
+            return level[key]
+
+        for k in ops:
+            lev(k)
+        # rounds: (level, kind) groups of <= 64 operations
+        groups = {}
+        for k, op in ops.items():
+            if op is not None:
+                groups.setdefault((level[k], op[0]), []).append(k)
+        vrounds = []
+        for g in sorted(groups):
+            ks = sorted(groups[g], key=lambda k: (k[0], k[1], k[2]))
+            for c in range(0, len(ks), 64):
+                vrounds.append((g[1], ks[c:c + 64]))
+        rnd_of = {}
+        for r, (_, ks) in enumerate(vrounds):
+            for k in ks:
+                rnd_of[k] = r
+        # runs: one per (e, f), m contiguous from the lowest needed m
+        mlo, mhi = {}, {}
+        for (e, f, m) in ops:
+            mlo[(e, f)] = min(mlo.get((e, f), m), m)
+            mhi[(e, f)] = max(mhi.get((e, f), m), m)
+        runs = {}
+        for ef in mlo:
+            d = -1 if ef == ((0, 0, 0), (0, 0, 0)) else min(rnd_of[(ef[0], ef[1], m)] for m in range(mlo[ef], mhi[ef] + 1)
+                                                            if (ef[0], ef[1], m) in rnd_of)
+            runs[ef] = [mhi[ef] - mlo[ef] + 1, d, d]
+        nvr = len(vrounds)
+        for k, op in ops.items():
+            if op is None:
+                continue
+            _, _, _, _, s0, s2, s4 = op
+            for s in (s0, s2, s4):
+                if s is not None:
+                    ef = (s[0], s[1])
+                    runs[ef][2] = max(runs[ef][2], rnd_of[k])
+        for e in self.es:
+            for f in self.fs:
+                runs[(e, f)][2] = nvr          # read by the accumulation
+        assert runs[((0, 0, 0), (0, 0, 0))][0] == L + 1
+        voff, vpeak = self._alloc(runs, nvr + 1)
+        # ---- HRR graph (the recursion HRRGen emits)
+        eidx = {e: k for k, e in enumerate(self.es)}
+        fidx = {f: k for k, f in enumerate(self.fs)}
+        nf = len(self.fs)
+        hops = {}    # key -> (kind, i, xkey, ykey)
+
+        def bra(a, b, f):
+            key = ("b", a, b, f)
+            if key in hops:
+                return key
+            if sum(b) == 0:
+                hops[key] = ("acc", eidx[a] * nf + fidx[f])
+            else:
+                i = pick(b)
+                b1 = add(b, i, -1)
+                hops[key] = (0, i, bra(add(a, i, 1), b1, f), bra(a, b1, f))
+            return key
+
+        def ket(a, b, c, d):
+            key = ("k", a, b, c, d)
+            if key in hops:
+                return key
+            if sum(d) == 0:
+                hops[key] = ("alias", bra(a, b, c))
+            else:
+                i = pick(d)
+                d1 = add(d, i, -1)
+                hops[key] = (1, i, ket(a, b, add(c, i, 1), d1), ket(a, b, c, d1))
+            return key
+
+        la, lb, lc, ld = self.cls
+        nb, nc, nd = ncart(lb), ncart(lc), ncart(ld)
+        outs = {}
+        for ia, a in enumerate(comps(la)):
+            for ib, b in enumerate(comps(lb)):
+                for ic, c in enumerate(comps(lc)):
+                    for id_, d in enumerate(comps(ld)):
+                        outs[ket(a, b, c, d)] = ((ia * nb + ib) * nc + ic) * nd + id_
+
+        def resolve(key):
+            while hops[key][0] == "alias":
+                key = hops[key][1]
+            return key
+
+        hlevel = {}
+
+        def hlev(key):
+            key = resolve(key)
+            if key in hlevel:
+                return hlevel[key]
+            op = hops[key]
+            hlevel[key] = 0 if op[0] == "acc" else 1 + max(hlev(op[2]), hlev(op[3]))
+            return hlevel[key]
+
+        real = set()
+        for k in list(hops):
+            rk = resolve(k)
+            hlev(rk)
+            if hops[rk][0] != "acc":
+                real.add(rk)
+        final = {}
+        for k, idx in outs.items():
+            final[resolve(k)] = idx
+        has_hrr = len(real) > 0
+        if has_hrr:
+            assert all(resolve(k) in real for k in outs), "mixed HRR outputs"
+        hgroups = {}
+        for k in real:
+            hgroups.setdefault((hlevel[k], hops[k][0]), []).append(k)
+        hrounds = []
+        for g in sorted(hgroups):
+            ks = sorted(hgroups[g], key=str)
+            for c in range(0, len(ks), 64):
+                hrounds.append((g[1], ks[c:c + 64]))
+        hrnd = {}
+        for r, (_, ks) in enumerate(hrounds):
+            for k in ks:
+                hrnd[k] = r
+        hruns = {}
+        for k in real:
+            if k in final:
+                continue
+            hruns[k] = [1, hrnd[k], hrnd[k]]
+        for k in real:
+            for s in (hops[k][2], hops[k][3]):
+                s = resolve(s)
+                if s in hruns:
+                    hruns[s][2] = max(hruns[s][2], hrnd[k])
+        hoff, hpeak = self._alloc(hruns, len(hrounds) + 1)
+        # ---- layout
+        acc0 = COOP_ACC
+        out0 = acc0 + self.nacc if has_hrr else acc0
+        work0 = out0 + (self.nout if has_hrr else 0)
+        W = max(vpeak, hpeak)
+        region = work0 + W
+        assert region * 8 < 65536
+
+        def vslot(key):
+            e, f, m = key
+            return work0 + voff[(e, f)] + (m - mlo[(e, f)])
+
+        def b(slot):
+            return 8 * slot
+
+        words = []
+        idle_v = (b(COOP_ZERO), b(COOP_TRASH) << 16)
+        for kind, ks in vrounds:
+            row = []
+            for k in ks:
+                _, i, k1, k2, s0, s2, s4 = ops[k]
+                x = (b(vslot(s0)) | k1) | ((b(vslot(s2)) if s2 else b(COOP_ZERO)) | k2) << 16
+                y = (b(vslot(s4)) if s4 else b(COOP_ZERO)) | ((b(vslot(k)) | i) << 16)
+                row.append((x, y))
+            row += [(b(COOP_ZERO) | (b(COOP_ZERO) << 16), b(COOP_ZERO) | (b(COOP_TRASH) << 16))] * (64 - len(row))
+            words += row
+        nacc_r = (self.nacc + 63) // 64
+        tgt = [(e, f) for e in self.es for f in self.fs]
+        for r in range(nacc_r):
+            row = []
+            for t in range(r * 64, min(self.nacc, r * 64 + 64)):
+                e, f = tgt[t]
+                row.append((b(vslot((e, f, 0))), b(acc0 + t)))
+            row += [(b(COOP_ZERO), b(COOP_TRASH))] * (64 - len(row))
+            words += row
+
+        def hslot(key):
+            key = resolve(key)
+            op = hops[key]
+            if op[0] == "acc":
+                return acc0 + op[1]
+            if key in final:
+                return out0 + final[key]
+            return work0 + hoff[key]
+
+        for kind, ks in hrounds:
+            row = []
+            for k in ks:
+                _, i, xk, yk = hops[k]
+                row.append((b(hslot(xk)) | (b(hslot(yk)) << 16), b(hslot(k)) | i))
+            row += [(b(COOP_ZERO) | (b(COOP_ZERO) << 16), b(COOP_TRASH))] * (64 - len(row))
+            words += row
+        # one idle round at the end: the kernels load the next round's
+        # descriptors one round ahead
+        words += [(b(COOP_ZERO) | (b(COOP_ZERO) << 16), b(COOP_ZERO) | (b(COOP_TRASH) << 16))] * 64
+        self.info = {"vrr_rounds": len(vrounds), "acc_rounds": nacc_r, "hrr_rounds": len(hrounds),
+                     "region": region, "work": W, "vpeak": vpeak, "hpeak": hpeak,
+                     "base": vslot(((0, 0, 0), (0, 0, 0), 0))}
+        return vrounds, hrounds, words, out0
+
+    def emit(self):
+        la, lb, lc, ld = self.cls
+        k = "%d%d%d%d" % self.cls
+        vrounds, hrounds, words, out0 = self.build()
+        I = self.info
+        lines = ["// Generated by gen_eri.py -- do not edit.",
+                 "// class (%d%d|%d%d): wave-cooperative program, %d VRR + %d accumulation + %d HRR rounds, "
+                 "%d doubles of LDS per wavefront" % (la, lb, lc, ld, I["vrr_rounds"], I["acc_rounds"],
+                                                     I["hrr_rounds"], I["region"]),
+                 "static __device__ const uint2 coop_prog_%s[%d] = {" % (k, len(words))]
+        assert len(vrounds) <= 64 and len(hrounds) <= 128
+        for c in range(0, len(words), 4):
+            lines.append("  " + ", ".join("{0x%xu, 0x%xu}" % w for w in words[c:c + 4]) + ",")
+        lines += ["};",
+                 "template <>", "struct CoopProgram<%d, %d, %d, %d> {" % self.cls,
+                 "  __device__ static __forceinline__ const uint2* prog() { return coop_prog_%s; }" % k,
+                 "  static_assert(kCoopZero == %d && kCoopAcc == %d, \"LDS layout of gen_eri.py\");" % (COOP_ZERO, COOP_ACC),
+                 "  static constexpr int kVrrRounds = %d;" % I["vrr_rounds"],
+                 "  static constexpr int kAccRounds = %d;" % I["acc_rounds"],
+                 "  static constexpr int kHrrRounds = %d;" % I["hrr_rounds"],
+                 "  static constexpr int kRegion = %d;" % I["region"],
+                 "  static constexpr int kOut = %d;" % out0,
+                 "  static constexpr int kBase = %d;" % I["base"],
+                 "  static constexpr unsigned kVrrKind = 0x%xu;" % sum(kd << r for r, (kd, _) in enumerate(vrounds) if r < 32),
+                 "  static constexpr unsigned kVrrKindHi = 0x%xu;" % sum(kd << (r - 32) for r, (kd, _) in enumerate(vrounds) if r >= 32),
+                 "  static constexpr unsigned long long kHrrKind0 = 0x%xull;" % sum(kd << r for r, (kd, _) in enumerate(hrounds) if r < 64),
+                 "  static constexpr unsigned long long kHrrKind1 = 0x%xull;" % sum(kd << (r - 64) for r, (kd, _) in enumerate(hrounds) if 64 <= r < 128),
+                 "};"]
+        return "\n".join(lines) + "\n"
+
+
 KERNEL_TU = """// Generated by gen_eri.py -- do not edit.
-// Kernels of quartet class (%(a)d%(b)d|%(c)d%(d)d): fused ERI + J/K, coupling%(schw)s.
-#include "../eri_core.h"
+// Kernels of quartet class (%(a)d%(b)d|%(c)d%(d)d): fused ERI + J/K, coupling%(schw)s%(coopnote)s.
+#include "../eri_coop.h"
 
 namespace lowdin {
 #include "eri_%(k)s.inc"
 LOWDIN_ERI_CLASS(%(a)d, %(b)d, %(c)d, %(d)d)
-
+%(coopinc)s
 void launch_jk_%(k)s(const JKArgs& args, hipStream_t s) { launch_jk_t<%(a)d, %(b)d, %(c)d, %(d)d>(args, s); }
 void launch_jks_%(k)s(const JKArgs& args, hipStream_t s) { launch_jk_strip_t<%(a)d, %(b)d, %(c)d, %(d)d>(args, s); }
 void launch_cp_%(k)s(const CouplingArgs& args, hipStream_t s) { launch_cp_t<%(a)d, %(b)d, %(c)d, %(d)d>(args, s); }
-%(schwfn)s}  // namespace lowdin
+%(schwfn)s%(coopfn)s}  // namespace lowdin
 """
+
+# classes with a wave-cooperative J/K kernel (total L >= 5)
+COOP_MIN_L = 5
 
 
 STORE_TU = """// Generated by gen_eri.py -- do not edit.
@@ -338,9 +705,19 @@ def main(outdir):
                   "const double* norma, hipStream_t s) { launch_schwarz_t<%d, %d>(pairs, npairs, prim, boys, norma, s); }\n"
                   % (la, lb, la, lb)
                   if diag else "")
+        coop = la + lb + lc + ld >= COOP_MIN_L
+        coopinc = coopfn = ""
+        if coop:
+            with open(os.path.join(gdir, "c_%s.inc" % k), "w") as fh:
+                fh.write(CoopGen(la, lb, lc, ld).emit())
+            coopinc = '#include "c_%s.inc"\n' % k
+            coopfn = ("bool launch_jkc_%s(const JKArgs& args, hipStream_t s) { return launch_jk_coop_t<%d, %d, %d, %d>(args, s); }\n"
+                      "void coop_attr_%s() { coop_attr_t<%d, %d, %d, %d>(); }\n" % (k, la, lb, lc, ld, k, la, lb, lc, ld))
         with open(os.path.join(gdir, "k_%s.hip" % k), "w") as fh:
             fh.write(KERNEL_TU % {"a": la, "b": lb, "c": lc, "d": ld, "k": k,
-                                  "schw": ", Schwarz" if diag else "", "schwfn": schwfn})
+                                  "schw": ", Schwarz" if diag else "", "schwfn": schwfn,
+                                  "coopnote": ", wave-cooperative J/K" if coop else "",
+                                  "coopinc": coopinc, "coopfn": coopfn})
         with open(os.path.join(gdir, "g_%s.hip" % k), "w") as fh:
             fh.write(G12_TU % {"a": la, "b": lb, "c": lc, "d": ld, "k": k})
         with open(os.path.join(gdir, "s_%s.hip" % k), "w") as fh:
@@ -362,6 +739,9 @@ def main(outdir):
         lines.append("void launch_cst_%s(const CouplingStoreArgs&, hipStream_t);" % k)
         lines.append("void launch_gjk_%s(const G12Args&, hipStream_t);" % k)
         lines.append("void launch_gcp_%s(const G12CouplingArgs&, hipStream_t);" % k)
+        if la + lb + lc + ld >= COOP_MIN_L:
+            lines.append("bool launch_jkc_%s(const JKArgs&, hipStream_t);" % k)
+            lines.append("void coop_attr_%s();" % k)
     for (la, lb) in pcs:
         lines.append("void launch_schwarz_%d%d(PairInfo*, int, const double*, const double*, const double*, hipStream_t);"
                      % (la, lb))
@@ -375,6 +755,21 @@ def main(outdir):
     for (la, lb, lc, ld) in classes:
         lines.append("    case %d: launch_jks_%d%d%d%d(a, s); return true;" % (la * 1000 + lb * 100 + lc * 10 + ld, la, lb, lc, ld))
     lines.append("    default: return false;\n  }\n}")
+    lines.append("// wave-cooperative J/K kernel; false when the class has none or the launch")
+    lines.append("// does not fit in LDS")
+    lines.append("inline bool dispatch_jk_coop(int la, int lb, int lc, int ld, const JKArgs& a, hipStream_t s) {")
+    lines.append("  switch (la * 1000 + lb * 100 + lc * 10 + ld) {")
+    for (la, lb, lc, ld) in classes:
+        if la + lb + lc + ld >= COOP_MIN_L:
+            lines.append("    case %d: return launch_jkc_%d%d%d%d(a, s);" % (la * 1000 + lb * 100 + lc * 10 + ld, la, lb, lc, ld))
+    lines.append("    default: return false;\n  }\n}")
+    lines.append("inline bool has_jk_coop(int la, int lb, int lc, int ld) { return la + lb + lc + ld >= %d; }" % COOP_MIN_L)
+    lines.append("// raise the dynamic LDS limit of every wave-cooperative kernel (once per process)")
+    lines.append("inline void coop_attr_all() {")
+    for (la, lb, lc, ld) in classes:
+        if la + lb + lc + ld >= COOP_MIN_L:
+            lines.append("  coop_attr_%d%d%d%d();" % (la, lb, lc, ld))
+    lines.append("}")
     lines.append("inline bool dispatch_cp(int la, int lb, int lc, int ld, const CouplingArgs& a, hipStream_t s) {")
     lines.append("  switch (la * 1000 + lb * 100 + lc * 10 + ld) {")
     for (la, lb, lc, ld) in classes:

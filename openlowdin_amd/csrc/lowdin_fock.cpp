@@ -138,6 +138,7 @@ static void ensure_context() {
   std::vector<float> tabf(tab.begin(), tab.end());   // FP32 copy of the full table (mixed precision)
   c.boysf.upload(tabf.data(), tabf.size(), c.stream);
   HIPCHK(hipStreamSynchronize(c.stream));
+  coop_attr_all();
   c.ready = true;
 }
 
@@ -170,7 +171,8 @@ struct TaskSet {
     int cbra, cket;   // pair classes of bra / ket lists
     int bra_is_a;     // coupling: bra list belongs to the first species
     size_t off, count;
-    int strip;        // intra: one workgroup per task with LDS row strips (1: 256 threads, 2: 64)
+    int strip;        // intra: one workgroup per task with LDS row strips (1: 256 threads, 2: 64);
+                      // 3: wave-cooperative kernel with row strips, 4: the same without strips
   };
   std::vector<Launch> launches;
   std::vector<size_t> order;   // launch order: estimated cost, largest first
@@ -243,8 +245,10 @@ struct BuildPlan {
   DevBuf<int> d_ranges;                 // per class launch: first task, task count
   uint64_t layout = 0;                  // bumps whenever tasks or buffers move
   int target_wg[kNumClassPairs];        // workgroups a strip launch aims at (autotuned)
-  int kind[kNumClassPairs];             // 0: choose by ket run length, 1: LDS strips, 2: wavefront tasks,
-                                        // 3: LDS strips with one-wavefront workgroups
+  int kind[kNumClassPairs];             // 0: default, 1: LDS strips, 2: wavefront tasks,
+                                        // 3: LDS strips with one-wavefront workgroups,
+                                        // 4: wave-cooperative kernel with row strips (where LDS allows),
+                                        // 5: wave-cooperative kernel without strips
   lowdin_fock_stats stats{};
   std::vector<float> launch_ms;         // per class launch of the last build
   std::vector<unsigned long long> launch_cnt;
@@ -476,6 +480,9 @@ static void sort_by_cost(std::vector<Task>& t, size_t from, const PairInfo* bras
 
 // Largest LDS row-strip footprint a J/K workgroup may use.
 constexpr size_t kStripLdsMax = 65536;
+// Classes from this total angular momentum up use the wave-cooperative
+// kernel unless the autotuner chose otherwise.
+constexpr int kCoopDefaultL = 6;
 
 static inline int class_pair_id(int cx, int cy) { return cx * (cx + 1) / 2 + cy; }
 
@@ -537,13 +544,19 @@ static void build_plan_tasks(BuildPlan& p, hipStream_t st) {
       // LDS strips pay off when a bra has enough kets to fill a workgroup
       // (256 threads) for a while; short ket runs (small molecules) go to the
       // wavefront kernel, one bra x 64 kets per wave.
-      const int kd = p.kind[class_pair_id(cx, cy)];
+      int kd = p.kind[class_pair_id(cx, cy)];
+      // wave-cooperative kernel: by default from total L = kCoopDefaultL up
+      const bool coop_ok = has_jk_coop(la, lb, lc, ld);
+      const int coop = !coop_ok ? 0
+                       : (kd == 4 || (kd == 0 && la + lb + lc + ld >= kCoopDefaultL)) ? 3
+                       : kd == 5 ? 4 : 0;
+      if (kd >= 4) kd = 0;
       const bool strip_ok = (size_t)(ncart_h(la) + ncart_h(lb)) * p.nmax * sizeof(double) <= kStripLdsMax &&
                             maxbra <= kMaxStripPrims;
       // untuned: strips wherever they fit, of 256 threads when a bra has on
       // average >= 192 kets, else of one wavefront (short ket runs of small
       // molecules would leave most of a 256-thread workgroup idle)
-      const bool strip = strip_ok && (kd == 1 || kd == 3 || kd == 0);
+      const bool strip = coop || (strip_ok && (kd == 1 || kd == 3 || kd == 0));
       const bool wave64 = kd == 3 || (kd == 0 && total < 192 * nbras);
       const size_t from = ts.tasks.size();
       // run length: at least enough kets to amortise zeroing and flushing the
@@ -571,7 +584,8 @@ static void build_plan_tasks(BuildPlan& p, hipStream_t st) {
       }
       std::stable_sort(ts.tasks.begin() + from, ts.tasks.end(),
                        [&](const Task& x, const Task& y) { return task_cost(x, cx, cy) > task_cost(y, cx, cy); });
-      ts.launches.push_back({la, lb, lc, ld, cx, cy, 1, from, ts.tasks.size() - from, strip ? (wave64 ? 2 : 1) : 0});
+      ts.launches.push_back({la, lb, lc, ld, cx, cy, 1, from, ts.tasks.size() - from,
+                             coop ? coop : (strip ? (wave64 ? 2 : 1) : 0)});
       // estimated cost of the launch: primitive quartets x class weight
       const double w = 1.0 + (la + lb + lc + ld) * (la + lb + lc + ld);
       double c = 0.0;
@@ -698,7 +712,11 @@ static void ensure_launch_events(BuildPlan& p) {
   }
 }
 
-static inline bool heavy_launch(const TaskSet::Launch& L) { return L.la + L.lb + L.lc + L.ld >= kHeavyL; }
+// Per-lane kernels of the high-L classes spill to a large private segment;
+// the wave-cooperative kernels keep their state in LDS.
+static inline bool heavy_launch(const TaskSet::Launch& L) {
+  return L.strip < 3 && L.la + L.lb + L.lc + L.ld >= kHeavyL;
+}
 
 // Class launches of one build on the side streams, forked from and joined
 // back to st.  The scratch-heavy classes (few quartets, long per-lane
@@ -715,7 +733,10 @@ static void join_side_streams(hipStream_t st) {
   }
 }
 
-static void enqueue_classes(BuildPlan& p, hipStream_t st, bool timed, int which, bool join = true) {
+// mols / counts: descriptor array and per-task counter array the kernels
+// use (null: the plan's own; counts_off: no counters).
+static void enqueue_classes(BuildPlan& p, hipStream_t st, bool timed, int which, bool join = true,
+                            const MolDesc* mols = nullptr, bool counts_off = false) {
   Context& c = ctx();
   const size_t nl = p.ts.launches.size();
   bool any = false, any_heavy = false;
@@ -741,18 +762,21 @@ static void enqueue_classes(BuildPlan& p, hipStream_t st, bool timed, int which,
     JKArgs a{};
     a.tasks = p.ts.d_tasks.p + L.off;
     a.ntasks = (int)L.count;
-    a.mols = p.d_desc.p;
+    a.mols = mols ? mols : p.d_desc.p;
     a.boys = c.boys.p;
     a.boysf = c.boysf.p;
-    a.counts = p.d_task_counts.p + 3 * L.off;
+    a.counts = counts_off ? nullptr : p.d_task_counts.p + 3 * L.off;
     a.cbra = L.cbra;
     a.cket = L.cket;
     a.same_class = L.cbra == L.cket ? 1 : 0;
     a.lds_n = p.nmax;
     a.block = L.strip == 2 ? 64 : 256;
+    if (L.strip == 4) a.lds_n = 0;
     a.eps = 2.220446049250313080847e-16;  // DBL_EPSILON, Libint2Iface.h:269-271
     if (timed) HIPCHK(hipEventRecord(p.launch_ev[2 * k], ls));
-    const bool ok = L.strip ? dispatch_jk_strip(L.la, L.lb, L.lc, L.ld, a, ls) : dispatch_jk(L.la, L.lb, L.lc, L.ld, a, ls);
+    const bool ok = L.strip >= 3 ? dispatch_jk_coop(L.la, L.lb, L.lc, L.ld, a, ls)
+                    : L.strip    ? dispatch_jk_strip(L.la, L.lb, L.lc, L.ld, a, ls)
+                                 : dispatch_jk(L.la, L.lb, L.lc, L.ld, a, ls);
     if (!ok) throw std::runtime_error("no J/K kernel for class");
     if (timed) HIPCHK(hipEventRecord(p.launch_ev[2 * k + 1], ls));
   }
@@ -1938,7 +1962,8 @@ static int plan_autotune(lowdin::BuildPlan& p, const double* d_dens, double* d_r
   hipStream_t st = stream ? stream : c.stream;
   // candidates: the wavefront kernel, LDS strips aiming at that many
   // workgroups per launch, and LDS strips with one-wavefront workgroups
-  static const int cand[][2] = {{2, 2048}, {1, 512}, {1, 2048}, {1, 8192}, {3, 512}};   // {kind, workgroups}
+  static const int cand[][2] = {{2, 2048}, {1, 512}, {1, 2048}, {1, 8192}, {3, 512},
+                                {4, 512}, {4, 2048}, {5, 512}, {5, 2048}};   // {kind, workgroups}
   const bool tl = p.time_launches, se = p.serialize, ug = p.use_graph;
   p.time_launches = true;
   p.serialize = true;
@@ -1972,6 +1997,7 @@ static int plan_autotune(lowdin::BuildPlan& p, const double* d_dens, double* d_r
         const lowdin::TaskSet::Launch& L = p.ts.launches[k];
         const int id = lowdin::class_pair_id(L.cbra, L.cket);
         if (kind != 2 && !L.strip) continue;   // strips impossible for this class: candidate ran as waves
+        if ((kind >= 4) != (L.strip >= 3)) continue;   // no cooperative kernel, or a default one ran
         if (ms[k] < 0.97 * best[id]) {
           best[id] = ms[k];
           pick[id] = wg;
@@ -2027,17 +2053,27 @@ int lowdin_fock_autotune(void* lint, const double* d_dens, double* d_result, dou
   }
 }
 
+static int encode_target(int kind, int wg) {
+  switch (kind) {
+    case 2: return 0;
+    case 3: return -wg;
+    case 4: return 1000000 + wg;
+    case 5: return 2000000 + wg;
+    default: return wg;
+  }
+}
+
 int lowdin_fock_targets(void* lint, int* targets) {
   const lowdin::BuildPlan& p = lowdin::own_plan(*static_cast<Species*>(lint));
   for (int k = 0; k < lowdin::kNumClassPairs; ++k)
-    targets[k] = p.kind[k] == 2 ? 0 : (p.kind[k] == 3 ? -p.target_wg[k] : p.target_wg[k]);
+    targets[k] = encode_target(p.kind[k], p.target_wg[k]);
   return lowdin::kNumClassPairs;
 }
 
 int lowdin_fock_batch_targets(void* batch, int* targets) {
   const lowdin::BuildPlan& p = static_cast<lowdin_fock_batch_t*>(batch)->plan;
   for (int k = 0; k < lowdin::kNumClassPairs; ++k)
-    targets[k] = p.kind[k] == 2 ? 0 : (p.kind[k] == 3 ? -p.target_wg[k] : p.target_wg[k]);
+    targets[k] = encode_target(p.kind[k], p.target_wg[k]);
   return lowdin::kNumClassPairs;
 }
 

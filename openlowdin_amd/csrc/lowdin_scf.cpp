@@ -81,7 +81,9 @@ struct ScfSpecies {
   int nocc = 0;
   double eta = 2.0, charge = -1.0, factor = -0.5;
   bool build_two = true;
+  bool ghost = false;   // SCF_GHOST_SPECIES: its coupling stays out of the others' Fock matrices
   Buf H, S, X, D, G, Cs, R, F, T, Fp, W, Ework, C, E, Fh, Eh;
+  Buf Cg;               // coupling from ghost species (allocated only when there is one)
   bool have_onebody = false;
 };
 
@@ -211,6 +213,7 @@ void fock_begin(Scf& s, bool external_g) {
         throw std::runtime_error(std::string("two-particle build: ") + lowdin_fock_last_error());
     }
     lowdin::launch_fill_zero(p.Cs.p, nn, s.stream);
+    if (p.Cg.p) lowdin::launch_fill_zero(p.Cg.p, nn, s.stream);
   }
   // couplings: one pass per species pair gives both directions
   for (size_t a = 0; a < ns; ++a)
@@ -222,6 +225,10 @@ void fock_begin(Scf& s, bool external_g) {
       const double qq = pa.charge * pb.charge;
       lowdin::launch_sym_add(pa.Cs.p, pa.R.p, pa.n, qq, s.stream);
       lowdin::launch_sym_add(pb.Cs.p, pb.R.p, pb.n, qq, s.stream);
+      // the energy keeps every coupling (WaveFunction.f90:1980-1986); the
+      // Fock matrix of a species leaves out the ghost's (:1502-1503)
+      if (pb.ghost) lowdin::launch_sym_add(pa.Cg.p, pa.R.p, pa.n, qq, s.stream);
+      if (pa.ghost) lowdin::launch_sym_add(pb.Cg.p, pb.R.p, pb.n, qq, s.stream);
     }
   const int slot = s.nhist < kMaxDiis ? s.nhist : s.head;
   const int nstored = std::min(s.nhist + 1, kMaxDiis);
@@ -233,6 +240,7 @@ void fock_begin(Scf& s, bool external_g) {
     double* sc = s.d_sc + k * kScal;
     const double qq = p.charge * p.charge;
     lowdin::launch_fock_assemble(p.F.p, p.H.p, p.G.p, qq, p.Cs.p, nn, s.stream);
+    if (p.Cg.p) lowdin::launch_sym_add(p.F.p, p.Cg.p, n, -1.0, s.stream);
     BLASCHK(rocblas_ddot(s.blas, (rocblas_int)nn, p.D.p, 1, p.H.p, 1, sc + 0));
     BLASCHK(rocblas_ddot(s.blas, (rocblas_int)nn, p.D.p, 1, p.G.p, 1, sc + 1));
     BLASCHK(rocblas_ddot(s.blas, (rocblas_int)nn, p.D.p, 1, p.Cs.p, 1, sc + 2));
@@ -455,6 +463,7 @@ void* lowdin_scf_new(int nspecies, void* const* lints, const lowdin_scf_species*
       p.charge = desc[k].charge;
       p.factor = desc[k].factor;
       p.build_two = desc[k].build_two != 0;
+      p.ghost = desc[k].ghost != 0;
       const size_t nn = (size_t)p.n * p.n;
       for (Buf* b : {&p.H, &p.S, &p.X, &p.D, &p.G, &p.Cs, &p.R, &p.F, &p.T, &p.Fp, &p.C, &p.E}) b->alloc(nn);
       p.W.alloc(p.n);
@@ -463,6 +472,13 @@ void* lowdin_scf_new(int nspecies, void* const* lints, const lowdin_scf_species*
       p.Eh.alloc((size_t)kMaxDiis * nn);
       s->bytes += (12 + 2 * kMaxDiis) * nn * sizeof(double) + 2 * p.n * sizeof(double);
     }
+    bool any_ghost = false;
+    for (const ScfSpecies& p : s->sp) any_ghost = any_ghost || p.ghost;
+    if (any_ghost)
+      for (ScfSpecies& p : s->sp) {
+        p.Cg.alloc((size_t)p.n * p.n);
+        s->bytes += (size_t)p.n * p.n * sizeof(double);
+      }
     lowdin::clear_last_error();
     return s;
   } catch (const std::exception& e) {

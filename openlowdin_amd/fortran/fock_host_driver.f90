@@ -36,6 +36,7 @@ program fock_host_driver
   logical :: have
   real(8), allocatable :: gexp(:), gcoef(:)
   real(8), allocatable, target :: A3(:,:,:)
+  integer :: ghost_id
 
   call get_command_argument(1, dir)
   open(newunit=u, file=trim(dir)//"/job.txt", status="old", action="read")
@@ -59,6 +60,16 @@ program fock_host_driver
      deallocate(lam, npr, orig, ex, co)
   end do
   close(u)
+
+  ! optional ghost.txt: id of the SCF_GHOST_SPECIES, whose coupling is left
+  ! out of the other species' coupling matrices (WaveFunction.f90:1502-1503)
+  ghost_id = -1
+  inquire(file=trim(dir)//"/ghost.txt", exist=have)
+  if (have) then
+     open(newunit=u, file=trim(dir)//"/ghost.txt", status="old", action="read")
+     read(u, *) ghost_id
+     close(u)
+  end if
 
   do k = 1, ns
      allocate(dens(k)%v(nbas(k), nbas(k)))
@@ -89,7 +100,7 @@ program fock_host_driver
               aux(i2, i) = aux(i, i2)
            end do
         end do
-        coupling = coupling + aux
+        if (ids(j) /= ghost_id) coupling = coupling + aux
      end do
      call write_matrix(trim(dir)//"/coupling_"//itoa(ids(k))//".bin", coupling)
      deallocate(coupling)

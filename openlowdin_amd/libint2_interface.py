@@ -46,7 +46,7 @@ class Stats(C.Structure):
 class ScfSpeciesDesc(C.Structure):
     """lowdin_scf_species."""
     _fields_ = [("nocc", C.c_int), ("eta", C.c_double), ("charge", C.c_double), ("factor", C.c_double),
-                ("build_two", C.c_int)]
+                ("build_two", C.c_int), ("ghost", C.c_int)]
 
 
 def lib():

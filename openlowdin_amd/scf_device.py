@@ -19,7 +19,8 @@ from . import wavefunction as WF
 
 class DeviceSCF:
     def __init__(self, system, objs: Optional[Dict[int, LI.Libint2Interface]] = None,
-                 build_for_one_particle: bool = False, stream: int = 0, hcore=None, overlap=None):
+                 build_for_one_particle: bool = False, stream: int = 0, hcore=None, overlap=None,
+                 ghost: str = ""):
         self.system = system
         self.objs = {} if objs is None else objs
         ns = len(system.species)
@@ -33,6 +34,7 @@ class DeviceSCF:
             desc[k].charge = sp.charge
             desc[k].factor = sp.exchange_factor
             desc[k].build_two = 1 if (sp.n_particles > 1 or build_for_one_particle) else 0
+            desc[k].ghost = 1 if (ghost and sp.name == ghost) else 0
         arr = (C.c_void_p * ns)(*[h.this for h in self.handles])
         self.this = LI.lib().lowdin_scf_new(ns, arr, desc, float(system.point_charge_repulsion()), stream or None)
         if not self.this:

@@ -90,12 +90,38 @@ def _sym_orth(S, thresh=1e-8):
     return U[:, keep] / np.sqrt(w[keep])
 
 
+def damping_step(Finit, Dinit, Fnew, Dnew, zero: float = 1e-12):
+    """One step of the reference's damping (Convergence_dampingMethod,
+    Convergence.f90:444-515).  With dD = Dnew - Dinit, the density effect
+    -1/2 Tr(Finit dD) and the Fock-and-density effect Tr((Fnew - Finit) dD)
+    give the damping factor a = density / Fock-and-density; no damping when
+    the latter is not above the former or is below DOUBLE_ZERO_THRESHOLD
+    (1e-12).  Returns (Fock matrix to diagonalise, new Finit, new Dinit)."""
+    dD = Dnew - Dinit
+    de = -0.5 * float(np.sum(Finit * dD.T))
+    fde = float(np.sum((Fnew - Finit) * dD.T))
+    if fde <= de or abs(fde) < zero:
+        return Fnew, Fnew.copy(), Dnew.copy()
+    a = de / fde
+    F0 = Finit + a * (Fnew - Finit)
+    return F0, F0.copy(), Dinit + a * dD
+
+
 def run_scf(system, hcore: Optional[List[np.ndarray]] = None, overlap: Optional[List[np.ndarray]] = None,
             max_iter: int = 300,
             e_tol: float = 1e-11, d_tol: float = 1e-9, objs=None, build_for_one_particle: bool = False,
-            verbose: bool = False):
+            verbose: bool = False, guess=None, history=None, convergence: str = "diis", ghost: str = ""):
     """Multi-species HF with GPU Fock builds.  hcore/overlap per species
     (normalised Cartesian basis; computed on the GPU when not given).
+    guess: starting densities (None entries: H-core guess); history: list
+    receiving every iteration's energy.  convergence: "diis" (DIIS(8) over
+    all species, converged when |dE| < e_tol and the orthonormal FDS - SDF
+    error < d_tol) or "damping" (the reference's default, CONVERGENCE_METHOD=1
+    with ITERATION_SCHEME=3: one damped update of every species per global
+    iteration, Convergence.f90:444-515, converged when |dE| < e_tol, the
+    ENERGY criterion, MultiSCF.f90:795-830).  ghost: name of a species whose
+    coupling is left out of the other species' Fock matrices but kept in the
+    energy (SCF_GHOST_SPECIES, WaveFunction.f90:1502-1503, 1980-1986).
     Returns (energy, wavefunctions)."""
     objs = {} if objs is None else objs
     if hcore is None or overlap is None:
@@ -110,7 +136,10 @@ def run_scf(system, hcore: Optional[List[np.ndarray]] = None, overlap: Optional[
         return wfs[k].sp.eta * occ @ occ.T
 
     for k, wf in enumerate(wfs):
-        wf.density = density(k, hcore[k])
+        if guess is not None and guess[k] is not None:
+            wf.density = np.array(guess[k], dtype=np.float64)   # read coefficients (wfn_io)
+        else:
+            wf.density = density(k, hcore[k])
     e_pc = system.point_charge_repulsion()
     hist_F, hist_e = [], []
     e_old, energy = None, None
@@ -119,7 +148,7 @@ def run_scf(system, hcore: Optional[List[np.ndarray]] = None, overlap: Optional[
         Fs = []
         for k, wf in enumerate(wfs):
             G = wf.build_two_particles_matrix(build_for_one_particle=build_for_one_particle)
-            C = wf.build_coupling_matrix(dens)
+            C = wf.build_coupling_matrix(dens, ghost=ghost)
             Fs.append(hcore[k] + G + C)
         energy = e_pc
         for k, wf in enumerate(wfs):
@@ -131,6 +160,21 @@ def run_scf(system, hcore: Optional[List[np.ndarray]] = None, overlap: Optional[
         err = max(float(np.abs(e).max()) for e in errs)
         if verbose:
             print(it, energy, err)
+        if history is not None:
+            history.append(energy)
+        if convergence == "damping":
+            if e_old is not None and abs(energy - e_old) < e_tol:
+                break
+            e_old = energy
+            if it == 0:
+                damp = [(F.copy(), wf.density.copy()) for F, wf in zip(Fs, wfs)]
+            for k, wf in enumerate(wfs):
+                Fuse, Fi, Di = damping_step(damp[k][0], damp[k][1], Fs[k], wf.density)
+                damp[k] = (Fi, Di)
+                wf.density = density(k, Fuse)
+            continue
+        if convergence != "diis":
+            raise ValueError("convergence must be 'diis' or 'damping'")
         if e_old is not None and abs(energy - e_old) < e_tol and err < d_tol:
             break
         e_old = energy

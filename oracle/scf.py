@@ -82,11 +82,36 @@ def build_coupling(states, i):
     return Ctot, hart
 
 
+def damping_step(Finit, Dinit, Fnew, Dnew, zero=1e-12):
+    """Convergence_dampingMethod (Convergence.f90:444-515): the damping
+    factor a = dE_D / dE_FD from -1/2 Tr(F0 dD) and Tr(dF dD); returns the
+    Fock matrix to diagonalise and the new (Finit, Dinit)."""
+    dD = Dnew - Dinit
+    de = -0.5 * float(np.sum(Finit * dD.T))
+    fde = float(np.sum((Fnew - Finit) * dD.T))
+    if fde <= de or abs(fde) < zero:
+        return Fnew, Fnew.copy(), Dnew.copy()
+    a = de / fde
+    F0 = Finit + a * (Fnew - Finit)
+    return F0, F0.copy(), Dinit + a * dD
+
+
 def run_scf(system, max_iter: int = 300, e_tol: float = 1e-11, d_tol: float = 1e-9,
-            build_for_one: bool = False, verbose: bool = False):
+            build_for_one: bool = False, verbose: bool = False, guess=None, history=None,
+            convergence: str = "diis"):
+    """guess: optional list of starting densities (None entries: H-core);
+    history: optional list that receives the energy of every iteration.
+    convergence: "diis" (DIIS(8) on X^T (F D S - S D F) X, converged when
+    |dE| < e_tol and the error < d_tol) or "damping" (the reference's
+    default: CONVERGENCE_METHOD=1, ITERATION_SCHEME=3, one damped update of
+    every species per global iteration, converged when |dE| < e_tol, the
+    ENERGY criterion; MultiSCF.f90:276-440, 795-830)."""
     states = [SpeciesState(k + 1, sp, system.charges) for k, sp in enumerate(system.species)]
-    for st in states:
-        st.D, _ = st.density_from_fock(st.H)
+    for k, st in enumerate(states):
+        if guess is not None and guess[k] is not None:
+            st.D = np.array(guess[k], dtype=np.float64)
+        else:
+            st.D, _ = st.density_from_fock(st.H)
     e_pc = system.point_charge_repulsion()
     diis_F, diis_e = [], []
     e_old = None
@@ -107,6 +132,19 @@ def run_scf(system, max_iter: int = 300, e_tol: float = 1e-11, d_tol: float = 1e
         err = max(np.abs(e).max() for e in errs)
         if verbose:
             print(it, energy, err)
+        if history is not None:
+            history.append(energy)
+        if convergence == "damping":
+            if e_old is not None and abs(energy - e_old) < e_tol:
+                break
+            e_old = energy
+            if it == 0:
+                damp = [(F.copy(), st.D.copy()) for st, F in zip(states, Fs)]
+            for i, st in enumerate(states):
+                Fuse, Fi, Di = damping_step(damp[i][0], damp[i][1], Fs[i], st.D)
+                damp[i] = (Fi, Di)
+                st.D, _ = st.density_from_fock(Fuse)
+            continue
         if e_old is not None and abs(energy - e_old) < e_tol and err < d_tol:
             break
         e_old = energy

@@ -74,6 +74,16 @@ def test_fortran_driver_fhf_anion(tmp_path):
     up = np.triu(c12)
     expect = -(up + np.triu(up, 1).T)
     assert np.max(np.abs(cpl1 - expect)) < 1e-14
+    cpl2 = _read(str(tmp_path), "coupling_2.bin", p.nbasis)
+    # SCF_GHOST_SPECIES = the proton: the electrons' coupling leaves it out,
+    # the proton still feels the electrons (WaveFunction.f90:1502-1503)
+    with open(os.path.join(str(tmp_path), "ghost.txt"), "w") as fh:
+        fh.write("2\n")
+    print("[case] fortran driver: FHF- with the proton as ghost species", flush=True)
+    r = subprocess.run([DRIVER, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert np.max(np.abs(_read(str(tmp_path), "coupling_1.bin", e.nbasis))) == 0.0
+    assert np.max(np.abs(_read(str(tmp_path), "coupling_2.bin", p.nbasis) - cpl2)) < 1e-14
 
 
 @pytest.mark.gpu

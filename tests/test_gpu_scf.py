@@ -86,3 +86,68 @@ def test_reference_energy_device_scf(name, builder, eref):
     print("   E = %.12f  ref %.12f  diff %.2e  (%d iterations)" % (E, eref, E - eref, scf.iterations))
     assert scf.converged
     assert abs(E - eref) <= 1e-9
+
+
+def test_read_coefficients_energy_on_gpu():
+    """test/H.APMO.READCOEFF: the SCF started from the plainvec coefficients
+    the reference holds (tests/golden/, read as DensityMatrixSCFGuess
+    does) is at -0.464889091726 Eh from its first iteration."""
+    import os
+    from openlowdin_amd import wavefunction as WF
+    from openlowdin_amd import wfn_io as W
+    prefix = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "H.APMO.READCOEFF.")
+    sysm = S.build_system([("H", (0.0, 0.0, 0.0))], "CC-PVDZ",
+                          quantum=[("H_1", "DZSNB", (0.0, 0.0, 0.0), S.PROTON_MASS)], open_shell=True)
+    guess = []
+    for sp in sysm.species:
+        C, _ = W.read_coefficients(prefix, sp.name, sp.nbasis)
+        guess.append(W.guess_density(C, sp.n_occupied, sp.eta))
+    _case("GPU SCF from read coefficients: H.APMO.READCOEFF")
+    hist = []
+    E, _ = WF.run_scf(sysm, guess=guess, history=hist)
+    print("   E0 = %.12f  E = %.12f  iterations %d" % (hist[0], E, len(hist)))
+    assert abs(hist[0] - (-0.464889091726)) <= 1e-8
+    assert abs(E - (-0.464889091726)) <= 1e-9
+
+
+def test_reference_damping_convergence_on_gpu():
+    """test/H2O.APMO.iter3.py with the reference's default convergence
+    (damping, iteration scheme 3): -75.905888726296 Eh (1e-8) after
+    1069 +- 800 global iterations, every build on the GPU."""
+    from openlowdin_amd import wavefunction as WF
+    q = [("H_1", "NAKAI-3-SP", (0.0, 0.754175, 0.528381), S.PROTON_MASS),
+         ("H_1", "NAKAI-3-SP", (0.0, -0.754174, 0.528382), S.PROTON_MASS)]
+    _case("GPU SCF with damping: H2O.APMO.iter3")
+    hist = []
+    E, _ = WF.run_scf(S.build_system(H2O, "6-31G", quantum=q), max_iter=3000, e_tol=1e-10,
+                      convergence="damping", history=hist)
+    print("   E = %.12f after %d iterations (reference 1069)" % (E, len(hist)))
+    assert abs(E - (-75.905888726296)) <= 1e-8
+    assert abs(len(hist) - 1069) <= 800
+
+
+def test_ghost_species_device_equals_host():
+    """SCF_GHOST_SPECIES (WaveFunction.f90:1502-1503): with the FHF- proton
+    as ghost the electrons do not feel it, the proton still feels the
+    electrons, and the energy keeps every coupling (:1980-1986).  The
+    device-resident SCF and the host-driven SCF reach the same state."""
+    from openlowdin_amd import scf_device as SD
+    from openlowdin_amd import wavefunction as WF
+    build = lambda: S.build_system([("F", (0, 0, -1.140079)), ("H", (0, 0, 0)), ("F", (0, 0, 1.140079))], "6-31G",
+                                   quantum=[("H_1", "NAKAI-3-S", (0, 0, 0), 1.0078250321 * 1822.88850065855 - 1.0)],
+                                   extra_electrons=1)
+    _case("ghost species: FHF- with H_1 as ghost, device vs host SCF")
+    Eh, wfs = WF.run_scf(build(), ghost="H_1")
+    scf = SD.DeviceSCF(build(), ghost="H_1")
+    Ed = scf.run()
+    E0, _ = WF.run_scf(build())
+    print("   ghost: host %.12f device %.12f ; no ghost %.12f" % (Eh, Ed, E0))
+    assert scf.converged
+    assert abs(Ed - Eh) <= 1e-9
+    assert abs(Eh - E0) > 1e-4
+    # the electrons' Fock matrix holds no proton coupling: F_e = H_e + G_e
+    F = scf.matrix(0, "fock")
+    G = scf.matrix(0, "two")
+    Fref = F - G
+    assert np.max(np.abs(wfs[0].coupling)) == 0.0
+    assert np.max(np.abs(Fref - (Fref + Fref.T) / 2)) < 1e-12

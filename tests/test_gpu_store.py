@@ -39,7 +39,8 @@ def _store_close(got, ref, init, nprod):
     drop threshold, i.e. within rounding of the cut.  Every other element
     agrees to 1e-12."""
     got, ref = got.reshape(-1), ref.reshape(-1)
-    nprod = np.broadcast_to(nprod, got.shape).reshape(-1)
+    nprod = np.asarray(nprod, dtype=np.float64).reshape(-1)
+    assert nprod.shape == got.shape
     mism = (got == init) != (ref == init)
     kept = np.where(got == init, ref, got)
     raw = np.abs(kept[mism] / nprod[mism])

@@ -222,3 +222,22 @@ def test_reference_energy_hcn():
     import scf
     E, _ = scf.run_scf(S.build_system([("H", (0, 0, 0)), ("C", (0, 0, 1.059)), ("N", (0, 0, 2.186))], "6-311++G**"))
     assert abs(E - (-92.901743215468)) <= 1e-8
+
+
+@pytest.mark.slow
+def test_reference_damping_h2o_apmo_iter3():
+    """test/H2O.APMO.iter3.py: RHF electrons (6-31G) + two quantum protons
+    in one species (Nakai-3-sp), the reference's default convergence
+    (damping, iteration scheme 3, ENERGY criterion at 1e-10): energy
+    -75.905888726296 (1e-8) after 1069 (+-800) global iterations.  The
+    damping restated in oracle/scf.py converges in exactly 1069."""
+    import scf
+    h2o = [("O", (0.0, 0.0, -0.066575)), ("H", (0.0, 0.754175, 0.528381)), ("H", (0.0, -0.754174, 0.528382))]
+    q = [("H_1", "NAKAI-3-SP", (0.0, 0.754175, 0.528381), S.PROTON_MASS),
+         ("H_1", "NAKAI-3-SP", (0.0, -0.754174, 0.528382), S.PROTON_MASS)]
+    hist = []
+    E, _ = scf.run_scf(S.build_system(h2o, "6-31G", quantum=q), max_iter=3000, e_tol=1e-10,
+                       convergence="damping", history=hist)
+    assert abs(E - (-75.905888726296)) <= 1e-8
+    assert abs(len(hist) - 1069) <= 800
+    assert len(hist) == 1069
