@@ -226,6 +226,22 @@ int lowdin_fock_g12_device(void* lint, void* olint, const double* d_dens, double
 int lowdin_fock_compute_2body_device(void* lint, const double* d_dens, double* d_result, double factor,
                                      void* stream, int synchronize);
 
+/* lowdin_fock_compute_2body_device without any allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The build's scratch -- descriptors, the shell-block norms of
+ * D and the raw n x n accumulator -- lives in a file-scope device array of
+ * the library (5.8 MB, one copy per GPU, held until the process ends), so
+ * the basis is limited to 768 functions and 384 shells; larger ones are
+ * refused with an error.  The species must have been built once by
+ * LibintInterface_compute_2body_direct or lowdin_fock_compute_2body_device
+ * (that call prepares the geometry data) and not changed since.  The build
+ * is enqueued on `stream` (NULL: the library's stream) and not waited for;
+ * no statistics are kept.  Calls that overlap on one GPU share the scratch
+ * and race.  Returns 0 on success. */
+int lowdin_fock_compute_2body_device_noalloc(void* lint, const double* d_dens, double* d_result, double factor,
+                                             void* stream);
+
 /* Record HIP events around every class launch of the intra build. */
 int lowdin_fock_time_launches(void* lint, int enable);
 

@@ -886,6 +886,93 @@ static void enqueue_plan(BuildPlan& p, const double* dD, double* dG, const doubl
   HIPCHK(hipEventRecord(c.ev1, st));
 }
 
+// ------------------------------------------------ build without allocation
+// lowdin_fock_compute_2body_device_noalloc keeps the scratch of a build --
+// the molecule and symmetrisation descriptors, the shell-block norms of D
+// and the raw accumulator G_raw -- in this file-scope device array instead
+// of library allocations.  Every GPU holds its own copy until the process
+// ends; calls that overlap on one GPU share it and race.
+constexpr int kNoAllocMaxN = 768;        // basis functions
+constexpr int kNoAllocMaxShells = 384;   // shells
+constexpr size_t kNoAllocDescBytes = 512;
+constexpr size_t kNoAllocScratchBytes =
+    kNoAllocDescBytes + sizeof(double) * ((size_t)kNoAllocMaxShells * kNoAllocMaxShells +
+                                          (size_t)kNoAllocMaxN * kNoAllocMaxN);   // 5.8 MB
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static_assert(sizeof(MolDesc) + sizeof(SymDesc) <= kNoAllocDescBytes, "descriptor slots");
+
+// Device address of `scratch` on the current device (hipGetSymbolAddress
+// once per device; the host-side name of the array is not a device address).
+constexpr int kMaxDevices = 64;
+static void* g_scratch_dev[kMaxDevices];
+static char* scratch_ptr() {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device index out of range");
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_dev[dev] = ptr;
+  }
+  return static_cast<char*>(g_scratch_dev[dev]);
+}
+
+__global__ void store_desc_kernel(MolDesc* md, MolDesc m, SymDesc* sd, SymDesc y) {
+  *md = m;
+  *sd = y;
+}
+
+// The plan's tasks and every molecule's device data are current (what
+// ensure_plan would otherwise rebuild, allocating).
+static bool plan_ready(const BuildPlan& p) {
+  if (!p.valid || !p.ts.valid || p.sig_gen.size() != p.mols.size()) return false;
+  for (size_t m = 0; m < p.mols.size(); ++m) {
+    const Species& s = *p.mols[m];
+    if (!s.sig.valid || s.dev_generation != s.generation || p.sig_gen[m] != s.sig_gen) return false;
+  }
+  return true;
+}
+
+// One build of a single-species plan with its scratch in `scr` (the device
+// address of `scratch`): descriptors stored by a kernel, Dnorm, G_raw
+// zeroed by a fill kernel, the class launches (no counters, no events),
+// symmetrisation into dG.  Nothing is allocated and nothing waits.
+static void enqueue_noalloc(BuildPlan& p, Species& s, const double* dD, double* dG, double factor, hipStream_t st,
+                            char* scr) {
+  MolDesc* md = reinterpret_cast<MolDesc*>(scr);
+  SymDesc* sd = reinterpret_cast<SymDesc*>(scr + 256);
+  double* Dn = reinterpret_cast<double*>(scr + kNoAllocDescBytes);
+  double* Graw = Dn + (size_t)kNoAllocMaxShells * kNoAllocMaxShells;
+  MolDesc M;
+  std::memset(&M, 0, sizeof(M));
+  M.pairs = s.sig.d_pairs.p;
+  M.prim = s.sig.d_prims.p;
+  M.primf = s.sig.d_primsf.p;
+  M.D = dD;
+  M.Dn = Dn;
+  M.shells = s.d_shells.p;
+  M.norma = s.d_norma.p;
+  M.Graw = Graw;
+  for (int k = 0; k <= kNumPairClasses; ++k) M.off[k] = s.sig.off[k];
+  M.n = s.nbasis;
+  M.nshell = (int)s.shells.size();
+  M.screen = s.is_electron ? 1 : 0;
+  M.mixed = s.mixed;
+  M.half_factor = 0.5 * factor;
+  M.tau = s.tau;
+  SymDesc Y;
+  std::memset(&Y, 0, sizeof(Y));
+  Y.Graw = Graw;
+  Y.G = dG;
+  Y.n = s.nbasis;
+  hipLaunchKernelGGL(store_desc_kernel, dim3(1), dim3(1), 0, st, md, M, sd, Y);
+  if (M.screen || M.mixed) launch_dnorm_batch(md, 1, M.nshell, st);
+  launch_fill_zero(Graw, (size_t)M.n * M.n, st);
+  enqueue_classes(p, st, false, 0, true, md, true);
+  launch_symmetrize_batch(sd, 1, M.n, st);
+  HIPCHK(hipGetLastError());
+}
+
 // After the stream has drained: fold the counters and event times into stats.
 static void collect_plan_stats(BuildPlan& p, hipStream_t st, double t0) {
   Context& c = ctx();
@@ -1733,6 +1820,33 @@ int lowdin_fock_compute_2body_device(void* lint, const double* d_dens, double* d
     return 0;
   } catch (const std::exception& e) {
     lowdin::g_last_error = std::string("compute_2body_device: ") + e.what();
+    fprintf(stderr, "lowdin_fock: %s\n", lowdin::g_last_error.c_str());
+    return -1;
+  }
+}
+
+int lowdin_fock_compute_2body_device_noalloc(void* lint, const double* d_dens, double* d_result, double factor,
+                                             void* stream) {
+  Species* s = static_cast<Species*>(lint);
+  lowdin::Context& c = lowdin::ctx();
+  try {
+    std::lock_guard<std::mutex> lk(c.mu);
+    const int n = s->nbasis, ns = (int)s->shells.size();
+    if (n > lowdin::kNoAllocMaxN || ns > lowdin::kNoAllocMaxShells)
+      throw std::runtime_error("basis of " + std::to_string(n) + " functions / " + std::to_string(ns) +
+                               " shells exceeds the no-allocation scratch (at most " +
+                               std::to_string(lowdin::kNoAllocMaxN) + " functions and " +
+                               std::to_string(lowdin::kNoAllocMaxShells) + " shells)");
+    if (!c.ready || !s->own || !lowdin::plan_ready(*s->own))
+      throw std::runtime_error("the species has no current build plan: call LibintInterface_compute_2body_direct or "
+                               "lowdin_fock_compute_2body_device on it once first (that call prepares the geometry "
+                               "data; this entry point allocates nothing)");
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c.stream;
+    lowdin::enqueue_noalloc(*s->own, *s, d_dens, d_result, factor, st, lowdin::scratch_ptr());
+    lowdin::g_last_error.clear();
+    return 0;
+  } catch (const std::exception& e) {
+    lowdin::g_last_error = std::string("compute_2body_device_noalloc: ") + e.what();
     fprintf(stderr, "lowdin_fock: %s\n", lowdin::g_last_error.c_str());
     return -1;
   }

@@ -91,6 +91,8 @@ def lib():
         L.lowdin_fock_coupling_pair.restype = C.c_int
         L.lowdin_fock_compute_2body_device.argtypes = [vp, vp, vp, C.c_double, vp, C.c_int]
         L.lowdin_fock_compute_2body_device.restype = C.c_int
+        L.lowdin_fock_compute_2body_device_noalloc.argtypes = [vp, vp, vp, C.c_double, vp]
+        L.lowdin_fock_compute_2body_device_noalloc.restype = C.c_int
         L.lowdin_fock_time_launches.argtypes = [vp, C.c_int]
         L.lowdin_fock_serialize_launches.argtypes = [vp, C.c_int]
         L.lowdin_fock_1body.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp]
