@@ -38,7 +38,7 @@ def synthetic_density(n, nocc, seed=17):
 def cpu_baseline(sp, D, stride, eris_full):
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import oracle as O
-    threads = min(16, os.cpu_count() or 1)
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     b = O.Basis(sp)
     pairs, _ = O.shellpair_list(b)
     Q = O.schwarz(b)
@@ -52,6 +52,18 @@ def cpu_baseline(sp, D, stride, eris_full):
             "sample": "oracle (McMurchie-Davidson restatement of compute_2body_direct, OpenMP round-robin "
                       "over unique quartets) on every %d-th bra shell s1: %d of %d ERIs in %.1f s, scaled "
                       "by ERI count" % (stride, cnt["eris"], eris_full, dt)}
+
+
+def kernel_name(code, target, tuned):
+    """Kernel template that ran class `code` (e.g. "2121") under the
+    autotuner choice `target` (lowdin_fock_targets encoding).  Untuned plans
+    run the wave-cooperative kernel from total L = 6 and LDS strips below."""
+    args = ", ".join(code)
+    if target >= 1000000 or (not tuned and sum(int(c) for c in code) >= 6):
+        return "jk_coop_kernel<%s>" % args
+    if tuned and target == 0:
+        return "jk_kernel<%s>" % args
+    return "jk_strip_kernel<%s>" % args
 
 
 def time_device_builds(h, dD, dG, factor, stream, steps, H):
@@ -164,39 +176,75 @@ def batch_bench(steps, H, B, S, mixed_tau, stream):
     return out
 
 
-def batched_scf_bench(S, SV, mixed_tau):
-    """The 256 water clusters iterated to convergence together: one batched
-    Fock build (one launch per quartet class, HIP graph) per SCF tick, the
-    rest of each iteration on the device per molecule."""
+def batched_scf_bench(S, H, mixed_tau):
+    """The 256 water clusters iterated to convergence together in one
+    batched device-resident SCF (lowdin_scf_batch: every step of an
+    iteration one batched operation over all molecules, DIIS on the device,
+    HIP-graph replay while the set of molecules is unchanged); converged
+    molecules leave.  Against one batched Fock build of all 256."""
+    from openlowdin_amd import batch as B
+    from openlowdin_amd import scf_batch as SB
     systems = S.water_cluster_batch(256)
     out = {"molecules": len(systems)}
     for mode, tau in (("fp64", None), ("mixed", mixed_tau)):
-        eng = SV.ScfEngine(max_jobs=256, mixed_tau=tau)
-        for k, sysm in enumerate(systems):
-            eng.submit(SV.Job(k, "cluster%d" % k, sysm, 0.0))
+        eng = SB.BatchSCF(use_graph=True)
+        jobs = [eng.add(s, mixed_tau=tau) for s in systems]
+        live = dict(enumerate(jobs))
+        prev, its, conv = {}, {}, 0
+        step_ms = []
         t0 = time.perf_counter()
-        done = eng.run(timeout=900)
+        while live:
+            ts = time.perf_counter()
+            eng.step()
+            step_ms.append((time.perf_counter() - ts) * 1e3)
+            for idx, j in list(live.items()):
+                code, e, err, it, _ = eng.status(j)
+                ok = code == 0 and idx in prev and abs(e - prev[idx]) < 1e-9 and err < 1e-7
+                prev[idx] = e
+                if code != 0 or ok or it >= 200:
+                    conv += int(ok)
+                    its[idx] = it
+                    eng.remove(j)
+                    del live[idx]
         dt = time.perf_counter() - t0
-        out[mode] = {"seconds_to_converge_all": round(dt, 3), "ticks": eng.ticks,
-                     "fock_build_ms_per_tick": round(float(np.mean(eng.build_ms)), 3) if eng.build_ms else None,
-                     "fock_build_seconds_total": round(float(np.sum(eng.build_ms)) / 1e3, 3),
-                     "converged": int(sum(j.converged for j in done)),
-                     "mean_iterations": round(float(np.mean([j.iterations for j in done])), 2),
-                     "molecule_scf_per_s": round(len(done) / dt, 2)}
+        eng.close()
+        full = step_ms[2:6] if len(step_ms) > 6 else step_ms
+        out[mode] = {"seconds_to_converge_all": round(dt, 3), "steps": len(step_ms), "converged": conv,
+                     "mean_iterations": round(float(np.mean(list(its.values()))), 2),
+                     "molecule_scf_per_s": round(len(systems) / dt, 2),
+                     "step_ms_all_256_active": round(float(np.median(full)), 3)}
         if tau is not None:
             out[mode]["tau"] = tau
-        print("[bench] batched SCF %s: %d molecules in %.2f s (%d ticks)" % (mode, len(done), dt, eng.ticks),
-              file=sys.stderr, flush=True)
+        print("[bench] batched SCF %s: 256 molecules in %.2f s (%d steps, %.1f ms per step with all active)" % (
+            mode, dt, len(step_ms), out[mode]["step_ms_all_256_active"]), file=sys.stderr, flush=True)
+    # one batched build of the same 256 molecules (graph replay) for comparison
+    sps = [s.species[0] for s in systems]
+    fb = B.FockBatch.from_species(sps)
+    Ds = [synthetic_density(sp.nbasis, sp.n_occupied, seed=3 + k) for k, sp in enumerate(sps)]
+    dD = H.DeviceArray.from_host(fb.pack(Ds))
+    dG = H.DeviceArray((fb.packed_size,))
+    fb.options(use_graph=True)
+    stream = H.Stream()
+    fb.compute_device(dD.ptr, dG.ptr, stream.handle)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        fb.compute_device(dD.ptr, dG.ptr, stream.handle, synchronize=False)
+    stream.synchronize()
+    build_ms = (time.perf_counter() - t0) / 5 * 1e3
+    fb.close()
+    out["batched_build_ms_256"] = round(build_ms, 3)
+    out["fp64_step_over_build"] = round(out["fp64"]["step_ms_all_256_active"] / build_ms, 3)
     return out
 
 
 def serving_bench(S, SV, njobs, rel_rates, mixed_tau, seed=20261202):
     """Served SCF: Poisson arrivals of mixed jobs ((H2O)_1..24 / C2..C20
     alkanes, 6-31G** or cc-pVDZ, 20% with a quantum proton), continuous
-    batching under a 200 GB HBM budget.  The saturation rate is measured with
-    every job present at t = 0; arrival rates are fractions/multiples of it.
-    Serial FP64 solves (one job at a time) at the saturation rate are the
-    baseline."""
+    batching in size lanes under a 200 GB HBM budget.  The saturation rate is
+    measured with every job present at t = 0; arrival rates are
+    fractions/multiples of it.  Serial FP64 solves (one job at a time, one
+    lane) at the saturation rate are the baseline."""
     rng = np.random.default_rng(seed)
     jobs = [S.serving_job(rng) for _ in range(njobs)]
 
@@ -205,7 +253,9 @@ def serving_bench(S, SV, njobs, rel_rates, mixed_tau, seed=20261202):
         for k, ((label, sysm), t) in enumerate(zip(jobs, arrivals)):
             eng.submit(SV.Job(k, label, sysm, float(t)))
         eng.run(timeout=900)
-        return eng.report()
+        rep = eng.report()
+        eng.close()
+        return rep
 
     sat = run([0.0] * njobs)
     lam_sat = sat["jobs_per_s"]
@@ -227,9 +277,132 @@ def serving_bench(S, SV, njobs, rel_rates, mixed_tau, seed=20261202):
             rel, row["fp64"]["p50_s"], row["fp64"]["p99_s"], row["mixed"]["p50_s"], row["mixed"]["p99_s"]),
             file=sys.stderr, flush=True)
     arrivals = np.cumsum(gaps) / lam_sat
-    out["serial_fp64_at_saturation_rate"] = run(arrivals, max_jobs=1)
+    out["serial_fp64_at_saturation_rate"] = run(arrivals, max_jobs=1, lanes=1)
+    ser = out["serial_fp64_at_saturation_rate"]
+    one = next((r for r in out["rates"] if r["rate_rel_saturation"] == 1.0), None)
+    if one and ser.get("p99_s"):
+        out["served_over_serial_p99_at_saturation"] = round(one["fp64"]["p99_s"] / ser["p99_s"], 3)
+        out["served_over_serial_jobs_per_s"] = round(one["fp64"]["jobs_per_s"] / ser["jobs_per_s"], 3)
     if mixed_tau is not None:
         out["mixed_tau"] = mixed_tau
+    return out
+
+
+def small_configs_bench(H, LI, S, WF, steps, stream):
+    """The reference's CPU-runnable cases on the GPU (device buffers):
+      C0 water 6-31G** (n=25): build rate, and the first-iteration Fock
+         matrix F = H + G[D_hcore] against the same through the host entry;
+      C1 [F-H-F]- 6-311++G** electrons (n=46) + H_1 spd (n=40): the
+         electron and proton builds and the coupling in both directions
+         (one fused pass), per SCF iteration."""
+    out = {}
+    w = S.water("6-31G**")
+    sp = w.species[0]
+    n = sp.nbasis
+    hcore, overlap = WF.one_body_matrices(w, {})
+    X = WF._sym_orth(overlap[0])
+    ev, Cp = np.linalg.eigh(X.T @ hcore[0] @ X)
+    Cm = X @ Cp
+    D0 = 2.0 * Cm[:, :sp.n_occupied] @ Cm[:, :sp.n_occupied].T
+    h = LI.Libint2Interface(sp, 1)
+    h.init_2body_ints()
+    dD = H.DeviceArray.from_host(np.asfortranarray(D0).ravel(order="F"))
+    dG = H.DeviceArray((n * n,))
+    s0 = time_device_builds(h, dD, dG, -0.5, stream, max(steps, 20), H)
+    G1 = dG.download().reshape(n, n, order="F")
+    F1 = hcore[0] + G1
+    Gh = h.compute_2body_intraspecies_direct(D0, -0.5)
+    out["C0_water"] = {"n": n, "builds_per_s": round(1.0 / s0, 1), "ms_per_build": round(s0 * 1e3, 4),
+                       "eris_per_build": h.stats()["eris"],
+                       "first_iteration_fock": {"trace": float(np.trace(F1)), "max_abs": float(np.max(np.abs(F1))),
+                                                "max_abs_diff_vs_host_entry": float(np.max(np.abs(G1 - Gh)))}}
+    f = S.fhf_anion("6-311++G**")
+    e, pr = f.species
+    he, hp = LI.Libint2Interface(e, 1), LI.Libint2Interface(pr, 2)
+    he.init_2body_ints()
+    hp.init_2body_ints()
+    De = synthetic_density(e.nbasis, e.n_occupied)
+    Dp = synthetic_density(pr.nbasis, 1, seed=4) * 0.5
+    dDe = H.DeviceArray.from_host(np.ascontiguousarray(De))
+    dDp = H.DeviceArray.from_host(np.ascontiguousarray(Dp))
+    dGe, dGp = H.DeviceArray((e.nbasis ** 2,)), H.DeviceArray((pr.nbasis ** 2,))
+    dRe, dRp = H.DeviceArray((e.nbasis ** 2,)), H.DeviceArray((pr.nbasis ** 2,))
+    se = time_device_builds(he, dDe, dGe, -0.5, stream, max(steps, 20), H)
+    sp_ = time_device_builds(hp, dDp, dGp, -1.0, stream, max(steps, 20), H)
+    he.coupling_pair_device(hp, dDe.ptr, dDp.ptr, dRe.ptr, dRp.ptr, stream.handle)
+    reps = max(steps, 20)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        he.coupling_pair_device(hp, dDe.ptr, dDp.ptr, dRe.ptr, dRp.ptr, stream.handle, synchronize=False)
+    stream.synchronize()
+    sc = (time.perf_counter() - t0) / reps
+    out["C1_fhf"] = {"n_e": e.nbasis, "n_p": pr.nbasis, "electron_builds_per_s": round(1.0 / se, 1),
+                     "proton_builds_per_s": round(1.0 / sp_, 1), "coupling_pairs_per_s": round(1.0 / sc, 1),
+                     "ms_per_iteration_builds": round((se + sp_ + sc) * 1e3, 4)}
+    return out
+
+
+def coupling_bench(H, LI, S, steps, stream):
+    """The Coulomb coupling build (compute_coupling_direct,
+    Libint2Iface.cpp:1111-1294; every pair of both species, no screening):
+    C20H42 cc-pVDZ electrons with one quantum proton (H_1 spd basis of the
+    [F-H-F]- case) at the first terminal hydrogen, both directions in one
+    fused pass, device buffers."""
+    atoms = S.alkane_geometry(20)
+    hidx = next(k for k, (el, _) in enumerate(atoms) if el == "H")
+    sysm = S.with_quantum_proton(atoms, "cc-pVDZ", hidx)
+    e, pr = sysm.species
+    he, hp = LI.Libint2Interface(e, 1), LI.Libint2Interface(pr, 2)
+    De = synthetic_density(e.nbasis, e.n_occupied)
+    Dp = synthetic_density(pr.nbasis, 1, seed=4) * 0.5
+    dDe = H.DeviceArray.from_host(np.ascontiguousarray(De))
+    dDp = H.DeviceArray.from_host(np.ascontiguousarray(Dp))
+    dRe, dRp = H.DeviceArray((e.nbasis ** 2,)), H.DeviceArray((pr.nbasis ** 2,))
+    he.coupling_pair_device(hp, dDe.ptr, dDp.ptr, dRe.ptr, dRp.ptr, stream.handle)
+    st = he.stats()
+    reps = max(3, steps // 2)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        he.coupling_pair_device(hp, dDe.ptr, dDp.ptr, dRe.ptr, dRp.ptr, stream.handle, synchronize=False)
+    stream.synchronize()
+    sc = (time.perf_counter() - t0) / reps
+    ns_e, ns_p = len(e.shells), len(pr.shells)
+    quartets = (ns_e * (ns_e + 1) // 2) * (ns_p * (ns_p + 1) // 2)
+    return {"n_e": e.nbasis, "n_p": pr.nbasis, "ms_per_coupling_pair": round(sc * 1e3, 4),
+            "coupling_pairs_per_s": round(1.0 / sc, 2), "shell_quartets": quartets,
+            "eris": st.get("eris"), "eris_per_s": round(st["eris"] / sc, 1) if st.get("eris") else None}
+
+
+def variants_bench(h, dD, dG, stream, steps, H, G_ref):
+    """The headline build with the reference's per-call setup (shell-pair
+    list and Schwarz bounds recomputed before every build, as
+    Libint2Interface.f90:544 and Libint2Iface.cpp:3520 do) and with the
+    BASELINE wording's 1e-12 quartet screen instead of DBL_EPSILON."""
+    out = {}
+    h.set_setup_each_build(True)
+    h.compute_2body_device(dD.ptr, dG.ptr, -0.5, stream.handle, synchronize=True)
+    reps = max(3, steps // 2)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        h.init_2body_ints()
+        h.compute_2body_device(dD.ptr, dG.ptr, -0.5, stream.handle, synchronize=True)
+    s_ref = (time.perf_counter() - t0) / reps
+    h.set_setup_each_build(False)
+    G_asref = dG.download()
+    out["as_reference_setup"] = {"builds_per_s": round(1.0 / s_ref, 3), "ms_per_build": round(s_ref * 1e3, 3),
+                                 "max_abs_dG_vs_amortized": float(np.max(np.abs(G_asref - G_ref)))}
+    h.set_screen(1e-12)
+    h.compute_2body_device(dD.ptr, dG.ptr, -0.5, stream.handle, synchronize=True)
+    st = h.stats()
+    s12 = time_device_builds(h, dD, dG, -0.5, stream, steps, H)
+    G12 = dG.download()
+    h.set_screen(2.220446049250313e-16)
+    out["screen_1e-12"] = {"builds_per_s": round(1.0 / s12, 3), "ms_per_build": round(s12 * 1e3, 3),
+                           "quartets": st["quartets"], "eris": st["eris"],
+                           "max_abs_dG_vs_dbl_epsilon": float(np.max(np.abs(G12 - G_ref)))}
     return out
 
 
@@ -374,6 +547,10 @@ def main():
     ap.add_argument("--no-store", action="store_true", help="skip the integral store / transform runs")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default per-class kernel shapes")
     ap.add_argument("--no-g12", action="store_true", help="skip the Gaussian-geminal builds")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the per-call-setup and 1e-12-screen variants of the headline build")
+    ap.add_argument("--no-coupling", action="store_true", help="skip the coupling-build timing")
+    ap.add_argument("--no-small", action="store_true", help="skip the water / [F-H-F]- build rates")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 256 MiB device-to-device copy (profiler byte-counter calibration)")
     args = ap.parse_args()
@@ -455,6 +632,13 @@ def main():
                    if v[0] > 0 else None}
                for c, v in sorted(per_class.items(), key=lambda kv: -kv[1][0])}
 
+    ids = [(a, b) for a in range(6) for b in range(a + 1)]
+    pcl = [(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2)]
+    tmap = {}
+    targets = h.targets()
+    for k, (cx, cy) in enumerate(ids):
+        tmap["%d%d%d%d" % (pcl[cx] + pcl[cy])] = targets[k]
+    kname = kernel_name(code, tmap.get(code, 0), tuned is not None)
     # HBM traffic of the dominant kernel: measured offline with rocprofv3
     # FETCH_SIZE / WRITE_SIZE (copy-calibrated) and committed under profiles/
     traffic = None
@@ -462,7 +646,7 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as fh:
             tk = json.load(fh)["kernels"].get("jk_strip_kernel<%s>" % ", ".join(code))
-        if tk:
+        if tk and kname.startswith("jk_strip_kernel"):
             traffic = tk["fetch_bytes"] + tk["write_bytes"]
 
     out = {
@@ -487,7 +671,7 @@ def main():
         "prim_quartets_per_build": st["prim_quartets"],
         "event_ms_per_step": round(ev_ms / args.steps, 4),
         "build_tflops_fp64": round(total_flops * builds_per_s / 1e12, 3),
-        "roofline": {"bound": "valu", "kernel": "jk_strip_kernel<%s>" % ",".join(code),
+        "roofline": {"bound": "valu", "kernel": kname,
                      "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic, "traffic_unit": "bytes/launch",
                      "traffic_source": "profiles/jk_c20h42_fp64_traffic.md" if traffic else None, "flops_per_launch": fl,
@@ -497,6 +681,17 @@ def main():
         "serialized_ms_per_build": round(sum(v[0] for v in per_class.values()) / nprof, 4),
         "classes": classes,
     }
+    G_ref = dG.download()
+    if not args.no_variants:
+        print("[bench] per-call setup and 1e-12 screen variants...", file=sys.stderr, flush=True)
+        out["variants"] = variants_bench(h, dD, dG, stream, args.steps, H, G_ref)
+    if not args.no_coupling:
+        print("[bench] coupling build...", file=sys.stderr, flush=True)
+        out["coupling"] = coupling_bench(H, LI, S, args.steps, stream)
+    if not args.no_small:
+        from openlowdin_amd import wavefunction as WF
+        print("[bench] water / [F-H-F]- build rates...", file=sys.stderr, flush=True)
+        out["small_configs"] = small_configs_bench(H, LI, S, WF, args.steps, stream)
     if not args.no_mixed:
         from openlowdin_amd import wavefunction as WF
         print("[bench] C20H42 FP64 vs mixed SCF sweep...", file=sys.stderr, flush=True)
@@ -512,7 +707,7 @@ def main():
         from openlowdin_amd import serving as SV
         if not args.no_batch_scf:
             print("[bench] 256-cluster batched SCF...", file=sys.stderr, flush=True)
-            out["batched_scf"] = batched_scf_bench(S, SV, tau_best)
+            out["batched_scf"] = batched_scf_bench(S, H, tau_best)
         if not args.no_serving:
             print("[bench] served SCF...", file=sys.stderr, flush=True)
             out["serving"] = serving_bench(S, SV, args.serving_jobs,

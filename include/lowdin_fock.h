@@ -255,6 +255,19 @@ int lowdin_fock_time_launches(void* lint, int enable);
  * (dd|dd) always run in FP64.  Returns 0 on success. */
 int lowdin_fock_set_precision(void* lint, int mode, double tau);
 
+/* Quartet screen threshold of the species' intra builds (Dnorm * Q_ab * Q_cd
+ * < eps skips the quartet; electrons only).  Default DBL_EPSILON, the
+ * reference's (Libint2Iface.cpp:523-534); 1e-12 is the looser threshold the
+ * benchmark also reports.  Returns 0 on success. */
+int lowdin_fock_set_screen(void* lint, double eps);
+
+/* With enable != 0 every intra build first recomputes the pair records,
+ * the Schwarz bounds and the task lists, as the reference recomputes its
+ * pair list and Schwarz matrix on every call (Libint2Interface.f90:544,
+ * Libint2Iface.cpp:3520); the default reuses them while the shells are
+ * unchanged. */
+int lowdin_fock_set_setup_each_build(void* lint, int enable);
+
 /* Run the class launches of the intra build one after another on one
  * stream instead of overlapping them on several (for per-class timing). */
 int lowdin_fock_serialize_launches(void* lint, int enable);
@@ -382,6 +395,51 @@ int lowdin_scf_buffers(void* scf, int species, double** d_density, double** d_tw
 int64_t lowdin_scf_device_bytes(void* scf);
 /* The stream the SCF object works on (hipStream_t). */
 void* lowdin_scf_stream(void* scf);
+
+/* ------------------------------------------------ batched SCF
+ * Many independent SCF jobs (molecules of one or more species) iterated
+ * together on one stream, each step of an iteration one batched operation
+ * over all jobs: the direct builds of every species in one class launch per
+ * quartet class, couplings per species pair, and -- per group of species of
+ * equal basis size -- strided-batched GEMMs (error, X^T F X, C, D), one
+ * rocsolver dsyevd_strided_batched, and batched kernels for the Fock
+ * assembly, traces, error maxima and the DIIS history, B matrix, solve and
+ * extrapolation (the DIIS state stays on the device).  While the jobs are
+ * unchanged an iteration replays two captured HIP graphs around the
+ * diagonalisation.  Same iteration as lowdin_scf_* (no ghost species).  No
+ * reference counterpart: openLOWDIN iterates one molecule per process. */
+void* lowdin_scf_batch_new(void* stream);
+void lowdin_scf_batch_del(void* eng);
+/* Add a job: species handles (shells added, init_2body_ints called),
+ * descriptors, point-charge repulsion, per species host H and S (n x n,
+ * column-major).  Computes the orthogonaliser and the H-core guess (the
+ * call waits).  Returns the job id (>= 0) or -1. */
+int lowdin_scf_batch_add(void* eng, int nspecies, void* const* lints, const lowdin_scf_species* desc, double e_pc,
+                         const double* const* H, const double* const* S);
+/* Remove a job (its slots are reused by later jobs).  Returns 0 on success. */
+int lowdin_scf_batch_remove(void* eng, int job);
+/* use_graph != 0 (default): replay captured graphs while the jobs are
+ * unchanged.  inline_launch != 0: every kernel of the engine runs on its own
+ * stream (the default forks the class launches over the library's side
+ * streams); engines on different streams then overlap freely. */
+int lowdin_scf_batch_options(void* eng, int use_graph, int inline_launch);
+/* One SCF iteration of every job; waits for the per-job scalars.  Returns 0
+ * on success (a job whose diagonalisation failed is marked, the others go on). */
+int lowdin_scf_batch_step(void* eng);
+/* lowdin_scf_batch_step in halves: _begin enqueues (nothing waits), _query
+ * returns 1 once the iteration has finished on the device (0 otherwise),
+ * _end waits and collects.  Jobs are added and removed only between
+ * iterations. */
+int lowdin_scf_batch_step_begin(void* eng);
+int lowdin_scf_batch_step_query(void* eng);
+int lowdin_scf_batch_step_end(void* eng);
+/* Energy and error of the densities the job's last iteration started from,
+ * iterations done.  Returns 0 running, 1 failed (message in
+ * lowdin_fock_last_error), 2 removed, -1 no such job. */
+int lowdin_scf_batch_status(void* eng, int job, double* energy, double* error, int* iterations);
+/* what: 0 density, 1 Fock (extrapolated), 2 two-particle G, 3 coupling sum. */
+int lowdin_scf_batch_get(void* eng, int job, int species, int what, double* out);
+int64_t lowdin_scf_batch_device_bytes(void* eng);
 
 #ifdef __cplusplus
 }

@@ -42,6 +42,24 @@ __device__ __forceinline__ double& lds_ref(double* R, unsigned byteoff) {
   return *reinterpret_cast<double*>(reinterpret_cast<char*>(R) + byteoff);
 }
 
+// Independent rounds a lane takes at once (COOP_GROUP of gen_eri.py).
+constexpr int kCoopGroup = 8;
+
+// Group g of a stage: 4 bits (rounds - 1, kind << 3) packed 16 per word.
+template <unsigned long long G0, unsigned long long G1>
+__device__ __forceinline__ unsigned coop_code(int g) {
+  return (unsigned)(((g < 16 ? G0 : G1) >> (4 * (g & 15))) & 15ull);
+}
+
+// This lane's descriptors of group g, whose first round q points at.
+template <unsigned long long G0, unsigned long long G1>
+__device__ __forceinline__ void coop_load(const uint2* q, int g, uint2* d) {
+  const int nr = (int)(coop_code<G0, G1>(g) & 7u) + 1;
+#pragma unroll
+  for (int k = 0; k < kCoopGroup; ++k)
+    if (k < nr) d[k] = q[k * 64];
+}
+
 // Doubles of one wavefront's region: the program region, then the six
 // normalised density blocks (ab, cd, ac, bd, ad, bc), rounded up to even.
 template <int LA, int LB, int LC, int LD>
@@ -50,13 +68,156 @@ __host__ __device__ constexpr int coop_wave_doubles() {
   return (CoopProgram<LA, LB, LC, LD>::kRegion + NA * NB + NC * ND + NA * NC + NB * ND + NA * ND + NB * NC + 1) & ~1;
 }
 
+// One contracted quartet of class (LA LB|LC LD), raw (unnormalised
+// functions), by one wavefront: the [e0|f0] accumulators are cleared, every
+// primitive quartet (bra primitive pairs bprim[0..nbp), ket kprim[0..nkp))
+// runs the VRR rounds and adds its [e0|f0]^(0) targets, then the HRR rounds
+// leave (ab|cd) at R + kOut.  pg: this lane's column of the round table.
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void coop_quartet(double* R, const uint2* pg, gcptr<double> bprim, int nbp,
+                                             gcptr<double> kprim, int nkp, const double* boys, const PairInfo& P,
+                                             const PairInfo& K, int lane) {
+  using Pg = CoopProgram<LA, LB, LC, LD>;
+  constexpr int L = LA + LB + LC + LD;
+  constexpr int NACC = EriClass<LA, LB, LC, LD>::nacc;
+  for (int k = lane; k < NACC; k += 64) R[kCoopAcc + k] = 0.0;
+  for (int kp = 0; kp < nkp; ++kp) {
+    const gcptr<double> kq = kprim + kp * kPrimPairStride;
+    const double q = kq[0], o2q = kq[1];
+    const double Qx = kq[2], Qy = kq[3], Qz = kq[4];
+    const double Kcd = kq[8];
+    for (int ip = 0; ip < nbp; ++ip) {
+      const gcptr<double> bp = bprim + ip * kPrimPairStride;
+      const double p = bp[0], o2p = bp[1];
+      const double PQx = bp[2] - Qx, PQy = bp[3] - Qy, PQz = bp[4] - Qz;
+      const double rs = rsqrt_nr(p + q);
+      const double inv = rs * rs;
+      const double rop = q * inv, roq = p * inv;
+      const double T = p * rop * (PQx * PQx + PQy * PQy + PQz * PQz);
+      const double pref = bp[8] * Kcd * rs;
+      double F[L + 1];
+      boys_f64<L>(T, boys, F);
+      if (lane == 0) {
+        R[0] = bp[5];
+        R[1] = -rop * PQx;
+        R[2] = bp[6];
+        R[3] = -rop * PQy;
+        R[4] = bp[7];
+        R[5] = -rop * PQz;
+        R[6] = kq[5];
+        R[7] = roq * PQx;
+        R[8] = kq[6];
+        R[9] = roq * PQy;
+        R[10] = kq[7];
+        R[11] = roq * PQz;
+      }
+#pragma unroll
+      for (int m = 0; m <= L; ++m)
+        if (lane == m) R[Pg::kBase + m] = F[m] * pref;
+      const double o2pq = 0.5 * inv;
+      // VRR: groups of independent rounds; a lane loads every descriptor
+      // and operand of a group before it writes, and the next group's
+      // descriptors while it computes
+      {
+        const uint2* q = pg;
+        uint2 dn[kCoopGroup];
+        coop_load<Pg::kVrrG[0], Pg::kVrrG[1]>(q, 0, dn);
+#pragma unroll 1
+        for (int g = 0; g < Pg::kVrrGroups; ++g) {
+          const unsigned c = coop_code<Pg::kVrrG[0], Pg::kVrrG[1]>(g);
+          const int nr = (int)(c & 7u) + 1;
+          uint2 d[kCoopGroup];
+#pragma unroll
+          for (int k = 0; k < kCoopGroup; ++k) d[k] = dn[k];
+          q += nr * 64;
+          if (g + 1 < Pg::kVrrGroups) coop_load<Pg::kVrrG[0], Pg::kVrrG[1]>(q, g + 1, dn);
+          double v[kCoopGroup];
+          if (c & 8u) {
+#pragma unroll
+            for (int k = 0; k < kCoopGroup; ++k)
+              if (k < nr) {
+                const unsigned s0 = d[k].x & 0xfff8u, k1 = d[k].x & 7u;
+                const unsigned s2 = (d[k].x >> 16) & 0xfff8u, k2 = (d[k].x >> 16) & 7u;
+                const unsigned i = (d[k].y >> 16) & 3u;
+                v[k] = R[6 + 2 * i] * lds_at(R, s0) + R[7 + 2 * i] * lds_at(R, s0 + 8) +
+                       ((double)k1 * o2q) * (lds_at(R, s2) - roq * lds_at(R, s2 + 8)) +
+                       ((double)k2 * o2pq) * lds_at(R, d[k].y & 0xfff8u);
+              }
+          } else {
+#pragma unroll
+            for (int k = 0; k < kCoopGroup; ++k)
+              if (k < nr) {
+                const unsigned s0 = d[k].x & 0xfff8u, k1 = d[k].x & 7u;
+                const unsigned s2 = (d[k].x >> 16) & 0xfff8u;
+                const unsigned i = (d[k].y >> 16) & 3u;
+                v[k] = R[2 * i] * lds_at(R, s0) + R[1 + 2 * i] * lds_at(R, s0 + 8) +
+                       ((double)k1 * o2p) * (lds_at(R, s2) - rop * lds_at(R, s2 + 8));
+              }
+          }
+#pragma unroll
+          for (int k = 0; k < kCoopGroup; ++k)
+            if (k < nr) lds_ref(R, (d[k].y >> 16) & 0xfff8u) = v[k];
+        }
+        // accumulate the [e0|f0]^(0) targets
+        coop_load<Pg::kAccG[0], Pg::kAccG[1]>(q, 0, dn);
+#pragma unroll 1
+        for (int g = 0; g < Pg::kAccGroups; ++g) {
+          const int nr = (int)(coop_code<Pg::kAccG[0], Pg::kAccG[1]>(g) & 7u) + 1;
+          uint2 d[kCoopGroup];
+#pragma unroll
+          for (int k = 0; k < kCoopGroup; ++k) d[k] = dn[k];
+          q += nr * 64;
+          if (g + 1 < Pg::kAccGroups) coop_load<Pg::kAccG[0], Pg::kAccG[1]>(q, g + 1, dn);
+          double v[kCoopGroup];
+#pragma unroll
+          for (int k = 0; k < kCoopGroup; ++k)
+            if (k < nr) v[k] = lds_at(R, d[k].y) + lds_at(R, d[k].x);
+#pragma unroll
+          for (int k = 0; k < kCoopGroup; ++k)
+            if (k < nr) lds_ref(R, d[k].y) = v[k];
+        }
+      }
+    }
+  }
+  // HRR groups
+  if (lane == 0) {
+    R[12] = P.ABx;
+    R[13] = P.ABy;
+    R[14] = P.ABz;
+    R[15] = K.ABx;
+    R[16] = K.ABy;
+    R[17] = K.ABz;
+  }
+  {
+    const uint2* q = pg + (Pg::kVrrRounds + Pg::kAccRounds) * 64;
+    uint2 dn[kCoopGroup];
+    coop_load<Pg::kHrrG[0], Pg::kHrrG[1]>(q, 0, dn);
+#pragma unroll 1
+    for (int g = 0; g < Pg::kHrrGroups; ++g) {
+      const unsigned c = coop_code<Pg::kHrrG[0], Pg::kHrrG[1]>(g);
+      const int nr = (int)(c & 7u) + 1;
+      const int cb = (c & 8u) ? 15 : 12;
+      uint2 d[kCoopGroup];
+#pragma unroll
+      for (int k = 0; k < kCoopGroup; ++k) d[k] = dn[k];
+      q += nr * 64;
+      if (g + 1 < Pg::kHrrGroups) coop_load<Pg::kHrrG[0], Pg::kHrrG[1]>(q, g + 1, dn);
+      double v[kCoopGroup];
+#pragma unroll
+      for (int k = 0; k < kCoopGroup; ++k)
+        if (k < nr) v[k] = lds_at(R, d[k].x & 0xffffu) + R[cb + (d[k].y & 3u)] * lds_at(R, d[k].x >> 16);
+#pragma unroll
+      for (int k = 0; k < kCoopGroup; ++k)
+        if (k < nr) lds_ref(R, d[k].y & 0xfff8u) = v[k];
+    }
+  }
+}
+
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(256, 2) void jk_coop_kernel(JKArgs a) {
   using Pg = CoopProgram<LA, LB, LC, LD>;
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
-  constexpr int L = LA + LB + LC + LD;
   constexpr int NAB = NA * NB, NCD = NC * ND;
-  constexpr int NACC = EriClass<LA, LB, LC, LD>::nacc;
   constexpr int kWave = coop_wave_doubles<LA, LB, LC, LD>();
   constexpr int oDab = Pg::kRegion, oDcd = oDab + NAB, oDac = oDcd + NCD, oDbd = oDac + NA * NC,
                 oDad = oDbd + NB * ND, oDbc = oDad + NA * ND;
@@ -105,94 +266,8 @@ __global__ __launch_bounds__(256, 2) void jk_coop_kernel(JKArgs a) {
     }
     myq += 1;
     myp += (unsigned int)(P.nprim * K.nprim);
-    for (int k = lane; k < NACC; k += 64) R[kCoopAcc + k] = 0.0;
     const gcptr<double> kprim = prim + (size_t)K.poff * kPrimPairStride;
-    for (int kp = 0; kp < K.nprim; ++kp) {
-      const gcptr<double> kq = kprim + kp * kPrimPairStride;
-      const double q = kq[0], o2q = kq[1];
-      const double Qx = kq[2], Qy = kq[3], Qz = kq[4];
-      const double Kcd = kq[8];
-      for (int ip = 0; ip < P.nprim; ++ip) {
-        const gcptr<double> bp = bprim + ip * kPrimPairStride;
-        const double p = bp[0], o2p = bp[1];
-        const double PQx = bp[2] - Qx, PQy = bp[3] - Qy, PQz = bp[4] - Qz;
-        const double rs = rsqrt_nr(p + q);
-        const double inv = rs * rs;
-        const double rop = q * inv, roq = p * inv;
-        const double T = p * rop * (PQx * PQx + PQy * PQy + PQz * PQz);
-        const double pref = bp[8] * Kcd * rs;
-        double F[L + 1];
-        boys_f64<L>(T, a.boys, F);
-        if (lane == 0) {
-          R[0] = bp[5];
-          R[1] = -rop * PQx;
-          R[2] = bp[6];
-          R[3] = -rop * PQy;
-          R[4] = bp[7];
-          R[5] = -rop * PQz;
-          R[6] = kq[5];
-          R[7] = roq * PQx;
-          R[8] = kq[6];
-          R[9] = roq * PQy;
-          R[10] = kq[7];
-          R[11] = roq * PQz;
-        }
-#pragma unroll
-        for (int m = 0; m <= L; ++m)
-          if (lane == m) R[Pg::kBase + m] = F[m] * pref;
-        const double o2pq = 0.5 * inv;
-        // VRR rounds (runtime loop, next round's descriptor loaded one round ahead)
-        uint2 nx = pg[0];
-        #pragma unroll 1
-        for (int r = 0; r < Pg::kVrrRounds; ++r) {
-          const uint2 d = nx;
-          nx = pg[(r + 1) * 64];
-          const bool ket = ((r < 32 ? (Pg::kVrrKind >> r) : (Pg::kVrrKindHi >> (r - 32))) & 1u) != 0;
-          const unsigned s0 = d.x & 0xfff8u, k1 = d.x & 7u;
-          const unsigned s2 = (d.x >> 16) & 0xfff8u, k2 = (d.x >> 16) & 7u;
-          const unsigned dst = (d.y >> 16) & 0xfff8u, i = (d.y >> 16) & 3u;
-          const double x0 = lds_at(R, s0), x1 = lds_at(R, s0 + 8);
-          const double x2 = lds_at(R, s2), x3 = lds_at(R, s2 + 8);
-          double v;
-          if (ket) {
-            const double x4 = lds_at(R, d.y & 0xfff8u);
-            v = R[6 + 2 * i] * x0 + R[7 + 2 * i] * x1 + ((double)k1 * o2q) * (x2 - roq * x3) +
-                ((double)k2 * o2pq) * x4;
-          } else {
-            v = R[2 * i] * x0 + R[1 + 2 * i] * x1 + ((double)k1 * o2p) * (x2 - rop * x3);
-          }
-          lds_ref(R, dst) = v;
-        }
-        // accumulate the [e0|f0]^(0) targets
-        #pragma unroll 1
-        for (int r = 0; r < Pg::kAccRounds; ++r) {
-          const uint2 d = nx;
-          nx = pg[(Pg::kVrrRounds + r + 1) * 64];
-          lds_ref(R, d.y) += lds_at(R, d.x);
-        }
-      }
-    }
-    // HRR rounds
-    if (lane == 0) {
-      R[12] = P.ABx;
-      R[13] = P.ABy;
-      R[14] = P.ABz;
-      R[15] = K.ABx;
-      R[16] = K.ABy;
-      R[17] = K.ABz;
-    }
-    {
-      const uint2* hp = pg + (Pg::kVrrRounds + Pg::kAccRounds) * 64;
-      uint2 nx = hp[0];
-      #pragma unroll 1
-      for (int r = 0; r < Pg::kHrrRounds; ++r) {
-        const uint2 d = nx;
-        nx = hp[(r + 1) * 64];
-        const bool ket = ((r < 64 ? (Pg::kHrrKind0 >> r) : (Pg::kHrrKind1 >> (r - 64))) & 1ull) != 0;
-        const unsigned dst = d.y & 0xfff8u, i = d.y & 3u;
-        lds_ref(R, dst) = lds_at(R, d.x & 0xffffu) + R[(ket ? 15 : 12) + i] * lds_at(R, d.x >> 16);
-      }
-    }
+    coop_quartet<LA, LB, LC, LD>(R, pg, bprim, P.nprim, kprim, K.nprim, a.boys, P, K, lane);
     // J/K contraction of the (ab|cd) block (Libint2Iface.cpp:562-586)
     const double* out = R + Pg::kOut;
     const double deg = sbra * (K.sa != K.sb ? 2.0 : 1.0) * ((a.same_class && j == t.bra) ? 1.0 : 2.0);
@@ -309,6 +384,92 @@ __global__ __launch_bounds__(256, 2) void jk_coop_kernel(JKArgs a) {
   }
 }
 
+// Inter-species Coulomb coupling (Libint2Iface.cpp:1111-1294) of one class
+// with one wavefront per quartet: the quartet as above, then the drop of
+// raw integrals below a.drop and the two contractions of coupling_scatter
+// (eri_core.h) split over the lanes -- lane k < NA NB sums result element
+// (a b) on the bra species over the ket density's element set, lane k <
+// NC ND element (c d) on the ket species over the bra density's.  A
+// workgroup owns one coupling task (a bra pair and up to 64 ket pairs);
+// its wavefronts take the kets in turn.  No private segment: the state of
+// a quartet lives in the wavefront's LDS region.
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(256, 2) void coupling_coop_kernel(CouplingArgs a) {
+  using Pg = CoopProgram<LA, LB, LC, LD>;
+  constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  constexpr int NAB = NA * NB, NCD = NC * ND;
+  constexpr int kWave = coop_wave_doubles<LA, LB, LC, LD>();
+  constexpr int oWb = Pg::kRegion, oWk = oWb + NAB;   // bra and ket density weights
+  static_assert(NAB <= 64 && NCD <= 64, "one result element per lane");
+  static_assert(oWk + NCD <= kWave, "weights inside the wavefront's region");
+  extern __shared__ double lds[];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int nw = blockDim.x >> 6;
+  const Task t = a.tasks[blockIdx.x];
+  const PairInfo P = load_pair((gcptr<PairInfo>)a.bra + t.bra);
+  double* R = lds + w * kWave;
+  if (lane < 2) R[kCoopZero + lane] = 0.0;
+  const bool bdiag = P.sa == P.sb;
+  // weights of the bra density (result on the ket species): element set
+  // ia <= ib within a diagonal shell pair, doubled off the function
+  // diagonal, normalised
+  if (a.rket && lane < NAB) {
+    const int ia = lane / NB, ib = lane % NB;
+    const int fa = P.bfa + ia, fb = P.bfb + ib;
+    double v = 0.0;
+    if (!bdiag || ia <= ib)
+      v = (fa != fb ? 2.0 : 1.0) * a.dbra[(size_t)fa * a.nbra + fb] * a.nbra_norma[fa] * a.nbra_norma[fb];
+    R[oWb + lane] = v;
+  }
+  const gcptr<double> bprim = (gcptr<double>)a.bprim + (size_t)P.poff * kPrimPairStride;
+  const uint2* pg = Pg::prog() + lane;
+  const double drop = a.drop;
+  for (int j = t.j0 + w; j < t.j1; j += nw) {
+    const PairInfo K = load_pair((gcptr<PairInfo>)a.ket + j);
+    const gcptr<double> kprim = (gcptr<double>)a.kprim + (size_t)K.poff * kPrimPairStride;
+    coop_quartet<LA, LB, LC, LD>(R, pg, bprim, P.nprim, kprim, K.nprim, a.boys, P, K, lane);
+    const double* out = R + Pg::kOut;
+    const bool kdiag = K.sa == K.sb;
+    if (a.rbra) {
+      if (lane < NCD) {
+        const int ic = lane / ND, id = lane % ND;
+        const int fc = K.bfa + ic, fd = K.bfb + id;
+        double v = 0.0;
+        if (!kdiag || ic <= id)
+          v = (fc != fd ? 2.0 : 1.0) * a.dket[(size_t)fc * a.nket + fd] * a.nket_norma[fc] * a.nket_norma[fd];
+        R[oWk + lane] = v;
+      }
+      if (lane < NAB) {
+        const int ia = lane / NB, ib = lane % NB;
+        if (!(bdiag && ia > ib)) {
+          double s = 0.0;
+          for (int k = 0; k < NCD; ++k) {
+            const double x = out[lane * NCD + k];
+            s += R[oWk + k] * (fabs(x) < drop ? 0.0 : x);
+          }
+          const int fa = P.bfa + ia, fb = P.bfb + ib;
+          const int r = fa > fb ? fa : fb, c = fa > fb ? fb : fa;
+          unsafeAtomicAdd(&a.rbra[(size_t)r * a.nbra + c], s * a.nbra_norma[fa] * a.nbra_norma[fb]);
+        }
+      }
+    }
+    if (a.rket && lane < NCD) {
+      const int ic = lane / ND, id = lane % ND;
+      if (!(kdiag && ic > id)) {
+        double s = 0.0;
+        for (int k = 0; k < NAB; ++k) {
+          const double x = out[k * NCD + lane];
+          s += R[oWb + k] * (fabs(x) < drop ? 0.0 : x);
+        }
+        const int fc = K.bfa + ic, fd = K.bfb + id;
+        const int r = fc > fd ? fc : fd, c = fc > fd ? fd : fc;
+        unsafeAtomicAdd(&a.rket[(size_t)r * a.nket + c], s * a.nket_norma[fc] * a.nket_norma[fd]);
+      }
+    }
+  }
+}
+
 // Dynamic LDS a cooperative workgroup may use (gfx950: 160 KB per workgroup,
 // less the kernel's static counters).
 constexpr size_t kCoopLdsMax = 163840 - 256;
@@ -317,6 +478,20 @@ template <int LA, int LB, int LC, int LD>
 static void coop_attr_t() {
   (void)hipFuncSetAttribute((const void*)jk_coop_kernel<LA, LB, LC, LD>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)kCoopLdsMax);
+  (void)hipFuncSetAttribute((const void*)coupling_coop_kernel<LA, LB, LC, LD>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kCoopLdsMax);
+}
+
+// One workgroup of up to four wavefronts per coupling task.
+template <int LA, int LB, int LC, int LD>
+static bool launch_cp_coop_t(const CouplingArgs& a, hipStream_t s) {
+  if (a.ntasks <= 0) return true;
+  constexpr size_t wave = (size_t)coop_wave_doubles<LA, LB, LC, LD>() * sizeof(double);
+  int W = 4;
+  while (W > 1 && W * wave > kCoopLdsMax) --W;
+  if (W * wave > kCoopLdsMax) return false;
+  hipLaunchKernelGGL((coupling_coop_kernel<LA, LB, LC, LD>), dim3(a.ntasks), dim3(64 * W), W * wave, s, a);
+  return true;
 }
 
 // One workgroup per task.  a.lds_n: row-strip width wanted (0: none);

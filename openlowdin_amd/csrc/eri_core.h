@@ -193,10 +193,12 @@ __device__ __forceinline__ void contracted_quartet_body(BP bprim, int nbp, KP kp
 // T = 40 the asymptotic form with upward recursion.
 template <int L>
 __device__ __forceinline__ void boys_f32(float T, const float* __restrict__ tab, float* F) {
-  if (T < (float)kBoysTmax) {
-    const int i = (int)(T * (float)kBoysInvH + 0.5f);
-    const float x = (float)i * (float)kBoysH - T;
-    const float* r = tab + i * kBoysCols;
+  static_assert(1 + L + kBoysF32Taylor - 1 < kBoysF32Cols, "FP32 Boys table lacks the Taylor columns of this L");
+  if (T < kBoysF32Tmax) {
+    // T < Tmax rounds to a row <= Tmax * InvH, the table's last row
+    const int i = (int)(T * kBoysF32InvH + 0.5f);
+    const float x = (float)i * kBoysF32H - T;
+    const float* r = tab + i * kBoysF32Cols;
     float s = r[1 + L + 4];
     s = fmaf(s, x * 0.25f, r[1 + L + 3]);
     s = fmaf(s, x * (1.0f / 3.0f), r[1 + L + 2]);

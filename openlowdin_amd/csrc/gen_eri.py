@@ -302,6 +302,32 @@ class HRRGen:
 #   ACC  x = src, y = dst                      acc[dst] += V[src]
 #   HRR  x = x | (y << 16), y = dst | i        V = V[x] + C_i V[y], C = AB (kind 0) or CD (kind 1)
 COOP_P, COOP_ZERO, COOP_TRASH, COOP_ACC = 0, 18, 20, 22
+# Rounds of one recursion level and kind are independent: a lane loads the
+# descriptors and operands of up to COOP_GROUP of them before computing.
+COOP_GROUP = 8   # kCoopGroup of eri_coop.h
+
+
+def coop_groups(rounds):
+    """Consecutive rounds with equal (level, kind), in pieces of at most
+    COOP_GROUP: [(kind, nrounds)]."""
+    out = []
+    prev = None
+    for kind, _, key in rounds:
+        if key == prev and out[-1][1] < COOP_GROUP:
+            out[-1] = (kind, out[-1][1] + 1)
+        else:
+            out.append((kind, 1))
+        prev = key
+    return out
+
+
+def pack_groups(groups):
+    """4 bits per group (rounds - 1, kind << 3), 16 groups per word."""
+    words = [0, 0]
+    assert len(groups) <= 32
+    for g, (kind, nr) in enumerate(groups):
+        words[g // 16] |= ((nr - 1) | (kind << 3)) << (4 * (g % 16))
+    return words
 
 
 class CoopGen:
@@ -426,9 +452,9 @@ class CoopGen:
         for g in sorted(groups):
             ks = sorted(groups[g], key=lambda k: (k[0], k[1], k[2]))
             for c in range(0, len(ks), 64):
-                vrounds.append((g[1], ks[c:c + 64]))
+                vrounds.append((g[1], ks[c:c + 64], g))
         rnd_of = {}
-        for r, (_, ks) in enumerate(vrounds):
+        for r, (_, ks, _) in enumerate(vrounds):
             for k in ks:
                 rnd_of[k] = r
         # runs: one per (e, f), m contiguous from the lowest needed m
@@ -528,9 +554,9 @@ class CoopGen:
         for g in sorted(hgroups):
             ks = sorted(hgroups[g], key=str)
             for c in range(0, len(ks), 64):
-                hrounds.append((g[1], ks[c:c + 64]))
+                hrounds.append((g[1], ks[c:c + 64], g))
         hrnd = {}
-        for r, (_, ks) in enumerate(hrounds):
+        for r, (_, ks, _) in enumerate(hrounds):
             for k in ks:
                 hrnd[k] = r
         hruns = {}
@@ -561,7 +587,7 @@ class CoopGen:
 
         words = []
         idle_v = (b(COOP_ZERO), b(COOP_TRASH) << 16)
-        for kind, ks in vrounds:
+        for kind, ks, _ in vrounds:
             row = []
             for k in ks:
                 _, i, k1, k2, s0, s2, s4 = ops[k]
@@ -589,19 +615,20 @@ class CoopGen:
                 return out0 + final[key]
             return work0 + hoff[key]
 
-        for kind, ks in hrounds:
+        for kind, ks, _ in hrounds:
             row = []
             for k in ks:
                 _, i, xk, yk = hops[k]
                 row.append((b(hslot(xk)) | (b(hslot(yk)) << 16), b(hslot(k)) | i))
             row += [(b(COOP_ZERO) | (b(COOP_ZERO) << 16), b(COOP_TRASH))] * (64 - len(row))
             words += row
-        # one idle round at the end: the kernels load the next round's
-        # descriptors one round ahead
-        words += [(b(COOP_ZERO) | (b(COOP_ZERO) << 16), b(COOP_ZERO) | (b(COOP_TRASH) << 16))] * 64
+        # idle rounds at the end: the kernels load the next group's
+        # descriptors one group ahead
+        words += [(b(COOP_ZERO) | (b(COOP_ZERO) << 16), b(COOP_ZERO) | (b(COOP_TRASH) << 16))] * (64 * COOP_GROUP)
         self.info = {"vrr_rounds": len(vrounds), "acc_rounds": nacc_r, "hrr_rounds": len(hrounds),
                      "region": region, "work": W, "vpeak": vpeak, "hpeak": hpeak,
                      "base": vslot(((0, 0, 0), (0, 0, 0), 0))}
+        self.groups = (coop_groups(vrounds), coop_groups([(0, None, "acc")] * nacc_r), coop_groups(hrounds))
         return vrounds, hrounds, words, out0
 
     def emit(self):
@@ -614,7 +641,7 @@ class CoopGen:
                  "%d doubles of LDS per wavefront" % (la, lb, lc, ld, I["vrr_rounds"], I["acc_rounds"],
                                                      I["hrr_rounds"], I["region"]),
                  "static __device__ const uint2 coop_prog_%s[%d] = {" % (k, len(words))]
-        assert len(vrounds) <= 64 and len(hrounds) <= 128
+        vg, ag, hg = self.groups
         for c in range(0, len(words), 4):
             lines.append("  " + ", ".join("{0x%xu, 0x%xu}" % w for w in words[c:c + 4]) + ",")
         lines += ["};",
@@ -627,10 +654,13 @@ class CoopGen:
                  "  static constexpr int kRegion = %d;" % I["region"],
                  "  static constexpr int kOut = %d;" % out0,
                  "  static constexpr int kBase = %d;" % I["base"],
-                 "  static constexpr unsigned kVrrKind = 0x%xu;" % sum(kd << r for r, (kd, _) in enumerate(vrounds) if r < 32),
-                 "  static constexpr unsigned kVrrKindHi = 0x%xu;" % sum(kd << (r - 32) for r, (kd, _) in enumerate(vrounds) if r >= 32),
-                 "  static constexpr unsigned long long kHrrKind0 = 0x%xull;" % sum(kd << r for r, (kd, _) in enumerate(hrounds) if r < 64),
-                 "  static constexpr unsigned long long kHrrKind1 = 0x%xull;" % sum(kd << (r - 64) for r, (kd, _) in enumerate(hrounds) if 64 <= r < 128),
+                 "  // groups of independent rounds: 4 bits each (rounds - 1, kind << 3)",
+                 "  static constexpr int kVrrGroups = %d;" % len(vg),
+                 "  static constexpr unsigned long long kVrrG[2] = {0x%xull, 0x%xull};" % tuple(pack_groups(vg)),
+                 "  static constexpr int kAccGroups = %d;" % len(ag),
+                 "  static constexpr unsigned long long kAccG[2] = {0x%xull, 0x%xull};" % tuple(pack_groups(ag)),
+                 "  static constexpr int kHrrGroups = %d;" % len(hg),
+                 "  static constexpr unsigned long long kHrrG[2] = {0x%xull, 0x%xull};" % tuple(pack_groups(hg)),
                  "};"]
         return "\n".join(lines) + "\n"
 
@@ -712,11 +742,13 @@ def main(outdir):
                 fh.write(CoopGen(la, lb, lc, ld).emit())
             coopinc = '#include "c_%s.inc"\n' % k
             coopfn = ("bool launch_jkc_%s(const JKArgs& args, hipStream_t s) { return launch_jk_coop_t<%d, %d, %d, %d>(args, s); }\n"
-                      "void coop_attr_%s() { coop_attr_t<%d, %d, %d, %d>(); }\n" % (k, la, lb, lc, ld, k, la, lb, lc, ld))
+                      "bool launch_cpc_%s(const CouplingArgs& args, hipStream_t s) { return launch_cp_coop_t<%d, %d, %d, %d>(args, s); }\n"
+                      "void coop_attr_%s() { coop_attr_t<%d, %d, %d, %d>(); }\n"
+                      % (k, la, lb, lc, ld, k, la, lb, lc, ld, k, la, lb, lc, ld))
         with open(os.path.join(gdir, "k_%s.hip" % k), "w") as fh:
             fh.write(KERNEL_TU % {"a": la, "b": lb, "c": lc, "d": ld, "k": k,
                                   "schw": ", Schwarz" if diag else "", "schwfn": schwfn,
-                                  "coopnote": ", wave-cooperative J/K" if coop else "",
+                                  "coopnote": ", wave-cooperative J/K and coupling" if coop else "",
                                   "coopinc": coopinc, "coopfn": coopfn})
         with open(os.path.join(gdir, "g_%s.hip" % k), "w") as fh:
             fh.write(G12_TU % {"a": la, "b": lb, "c": lc, "d": ld, "k": k})
@@ -741,6 +773,7 @@ def main(outdir):
         lines.append("void launch_gcp_%s(const G12CouplingArgs&, hipStream_t);" % k)
         if la + lb + lc + ld >= COOP_MIN_L:
             lines.append("bool launch_jkc_%s(const JKArgs&, hipStream_t);" % k)
+            lines.append("bool launch_cpc_%s(const CouplingArgs&, hipStream_t);" % k)
             lines.append("void coop_attr_%s();" % k)
     for (la, lb) in pcs:
         lines.append("void launch_schwarz_%d%d(PairInfo*, int, const double*, const double*, const double*, hipStream_t);"
@@ -770,6 +803,13 @@ def main(outdir):
         if la + lb + lc + ld >= COOP_MIN_L:
             lines.append("  coop_attr_%d%d%d%d();" % (la, lb, lc, ld))
     lines.append("}")
+    lines.append("// wave-cooperative coupling kernel; false when the class has none")
+    lines.append("inline bool dispatch_cp_coop(int la, int lb, int lc, int ld, const CouplingArgs& a, hipStream_t s) {")
+    lines.append("  switch (la * 1000 + lb * 100 + lc * 10 + ld) {")
+    for (la, lb, lc, ld) in classes:
+        if la + lb + lc + ld >= COOP_MIN_L:
+            lines.append("    case %d: return launch_cpc_%d%d%d%d(a, s);" % (la * 1000 + lb * 100 + lc * 10 + ld, la, lb, lc, ld))
+    lines.append("    default: return false;\n  }\n}")
     lines.append("inline bool dispatch_cp(int la, int lb, int lc, int ld, const CouplingArgs& a, hipStream_t s) {")
     lines.append("  switch (la * 1000 + lb * 100 + lc * 10 + ld) {")
     for (la, lb, lc, ld) in classes:

@@ -33,6 +33,19 @@ constexpr double kBoysH = 0.1;
 constexpr double kBoysInvH = 10.0;
 constexpr double kBoysTmax = 40.0;
 constexpr int kBoysRows = 401;
+static_assert(kBoysRows == (int)(kBoysTmax * kBoysInvH) + 1, "FP64 Boys table rows must cover [0, kBoysTmax]");
+
+// FP32 Boys table (mixed precision): its own grid constants and its own
+// layout (exp, then F_0..F_{mmax+4} for a fifth-order Taylor step), built
+// separately from the FP64 one, so that a change of the FP64 table's range
+// or columns cannot move FP32 reads off their table.
+constexpr int kBoysF32Taylor = 5;
+constexpr int kBoysF32Cols = 1 + kBoysMmax + kBoysF32Taylor;
+constexpr float kBoysF32H = 0.1f;
+constexpr float kBoysF32InvH = 10.0f;
+constexpr float kBoysF32Tmax = 40.0f;
+constexpr int kBoysF32Rows = 401;
+static_assert(kBoysF32Rows == (int)(kBoysF32Tmax * kBoysF32InvH) + 1, "FP32 Boys table rows must cover [0, Tmax]");
            // T = 0 .. 40 inclusive
 
 struct ShellInfo {

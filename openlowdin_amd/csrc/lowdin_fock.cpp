@@ -82,7 +82,7 @@ struct Context {
   hipEvent_t fork = nullptr, join[kSideStreams] = {};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   DevBuf<double> boys;    // full table (one-body integrals)
-  DevBuf<float> boysf;    // FP32 copy of the full table (mixed precision)
+  DevBuf<float> boysf;    // FP32 table of its own grid (mixed precision)
   std::mutex mu;  // one process stream: calls are serialised
 };
 
@@ -119,6 +119,21 @@ static std::vector<double> make_boys_table() {
   return t;
 }
 
+// FP32 table of the mixed-precision quartets: rows T_i = i * kBoysF32H,
+// e^{-T_i} then F_0..F_{kBoysF32Cols-2}, each rounded from FP64.
+static std::vector<float> make_boys_table_f32() {
+  std::vector<float> t((size_t)kBoysF32Rows * kBoysF32Cols);
+  const int M = kBoysF32Cols - 2;
+  double F[kBoysF32Cols];
+  for (int i = 0; i < kBoysF32Rows; ++i) {
+    const double T = i * (double)kBoysF32H;
+    boys_point(T, M, F);
+    t[(size_t)i * kBoysF32Cols] = (float)std::exp(-T);
+    for (int m = 0; m <= M; ++m) t[(size_t)i * kBoysF32Cols + 1 + m] = (float)F[m];
+  }
+  return t;
+}
+
 static void ensure_context() {
   Context& c = ctx();
   if (c.ready) return;
@@ -135,7 +150,9 @@ static void ensure_context() {
   HIPCHK(hipEventCreate(&c.ev1));
   std::vector<double> tab = make_boys_table();
   c.boys.upload(tab.data(), tab.size(), c.stream);
-  std::vector<float> tabf(tab.begin(), tab.end());   // FP32 copy of the full table (mixed precision)
+  std::vector<float> tabf = make_boys_table_f32();
+  if (tabf.size() != (size_t)kBoysF32Rows * kBoysF32Cols)
+    throw std::runtime_error("FP32 Boys table size does not match its grid constants");
   c.boysf.upload(tabf.data(), tabf.size(), c.stream);
   HIPCHK(hipStreamSynchronize(c.stream));
   coop_attr_all();
@@ -217,6 +234,8 @@ struct Species {
   int mixed = 0;            // mixed precision: FP32 quartets below tau
   int sig_mixed = 0;        // the precision mode the pair order of sig was chosen for
   double tau = 0.0;
+  double screen_eps = 2.220446049250313080847e-16;   // quartet screen (DBL_EPSILON, Libint2Iface.h:269-271)
+  bool setup_each_build = false;   // recompute pair records and Schwarz bounds before every build
 
   ~Species();
 };
@@ -255,11 +274,14 @@ struct BuildPlan {
   std::vector<hipEvent_t> launch_ev;
   bool time_launches = false;
   bool serialize = false;               // all class launches on one side stream (clean per-class timing)
+  bool inline_launch = false;           // class launches on the caller's stream (no side streams)
   bool use_graph = false;
+  double eps = 2.220446049250313080847e-16;   // quartet screen threshold of the plan's launches
   hipGraphExec_t graph = nullptr;
   uint64_t graph_layout = ~0ull;
   hipStream_t graph_stream = nullptr;
   bool graph_any_dn = false;
+  double graph_eps = 0.0;
 
   BuildPlan() {
     for (int k = 0; k < kNumClassPairs; ++k) {
@@ -745,8 +767,13 @@ static void enqueue_classes(BuildPlan& p, hipStream_t st, bool timed, int which,
     any_heavy = any_heavy || heavy_launch(p.ts.launches[k]);
   }
   if (!any) return;
-  HIPCHK(hipEventRecord(c.fork, st));
-  for (int q = 0; q < kSideStreams; ++q) HIPCHK(hipStreamWaitEvent(c.side[q], c.fork, 0));
+  // inline plans launch on the caller's stream itself (several batched
+  // SCF lanes then overlap without sharing the library's side streams)
+  const bool fork = !p.inline_launch;
+  if (fork) {
+    HIPCHK(hipEventRecord(c.fork, st));
+    for (int q = 0; q < kSideStreams; ++q) HIPCHK(hipStreamWaitEvent(c.side[q], c.fork, 0));
+  }
   const int first_light = (any_heavy && which == 0) ? 1 : 0;
   size_t nlight = 0;
   for (size_t pass = 0; pass < 2 * nl; ++pass) {
@@ -758,7 +785,7 @@ static void enqueue_classes(BuildPlan& p, hipStream_t st, bool timed, int which,
     if ((which == 1 && heavy) || (which == 2 && !heavy)) continue;
     int q = 0;
     if (!p.serialize && !heavy) q = first_light + (int)(nlight++ % (size_t)(kSideStreams - first_light));
-    hipStream_t ls = c.side[q];
+    hipStream_t ls = fork ? c.side[q] : st;
     JKArgs a{};
     a.tasks = p.ts.d_tasks.p + L.off;
     a.ntasks = (int)L.count;
@@ -772,7 +799,7 @@ static void enqueue_classes(BuildPlan& p, hipStream_t st, bool timed, int which,
     a.lds_n = p.nmax;
     a.block = L.strip == 2 ? 64 : 256;
     if (L.strip == 4) a.lds_n = 0;
-    a.eps = 2.220446049250313080847e-16;  // DBL_EPSILON, Libint2Iface.h:269-271
+    a.eps = p.eps;  // DBL_EPSILON by default (Libint2Iface.h:269-271)
     if (timed) HIPCHK(hipEventRecord(p.launch_ev[2 * k], ls));
     const bool ok = L.strip >= 3 ? dispatch_jk_coop(L.la, L.lb, L.lc, L.ld, a, ls)
                     : L.strip    ? dispatch_jk_strip(L.la, L.lb, L.lc, L.ld, a, ls)
@@ -780,7 +807,7 @@ static void enqueue_classes(BuildPlan& p, hipStream_t st, bool timed, int which,
     if (!ok) throw std::runtime_error("no J/K kernel for class");
     if (timed) HIPCHK(hipEventRecord(p.launch_ev[2 * k + 1], ls));
   }
-  if (join) join_side_streams(st);
+  if (join && fork) join_side_streams(st);
 }
 
 // Dnorm and zeroed G_raw (no memset: capturable).
@@ -803,13 +830,16 @@ static void enqueue_epilogue(BuildPlan& p, hipStream_t st) {
 // allocated once the plan's buffers exist; the molecule descriptors are
 // re-uploaded only when a pointer, factor or precision setting changed.
 static void enqueue_plan(BuildPlan& p, const double* dD, double* dG, const double* factors, hipStream_t st,
-                         const double* const* dD_each = nullptr, double* const* dG_each = nullptr) {
+                         const double* const* dD_each = nullptr, double* const* dG_each = nullptr,
+                         bool record = true) {
   Context& c = ctx();
   ensure_plan(p, st);
   const int nm = (int)p.mols.size();
   p.desc.resize(nm);
   p.sym.resize(nm);
   p.any_dn = false;
+  p.eps = nm > 0 ? p.mols[0]->screen_eps : p.eps;
+  for (int m = 1; m < nm; ++m) p.eps = std::min(p.eps, p.mols[m]->screen_eps);
   for (int m = 0; m < nm; ++m) {
     Species& s = *p.mols[m];
     MolDesc& M = p.desc[m];
@@ -846,14 +876,15 @@ static void enqueue_plan(BuildPlan& p, const double* dD, double* dG, const doubl
   }
   const bool timed = p.time_launches && !p.use_graph;
   if (timed) ensure_launch_events(p);
-  HIPCHK(hipEventRecord(c.ev0, st));
+  if (record) HIPCHK(hipEventRecord(c.ev0, st));
   if (p.use_graph) {
     // The graph holds the classes without a large private segment.  Kernels
     // in a graph run on the runtime's own streams (one per branch), so the
     // scratch-heavy classes stay outside it, on side stream 0: they are
     // enqueued after the prologue and before the graph launch, and joined
     // with it before the epilogue, so that they run beside the graph.
-    if (!p.graph || p.graph_layout != p.layout || p.graph_stream != st || p.graph_any_dn != p.any_dn) {
+    if (!p.graph || p.graph_layout != p.layout || p.graph_stream != st || p.graph_any_dn != p.any_dn ||
+        p.graph_eps != p.eps) {
       if (p.graph) HIPCHK(hipGraphExecDestroy(p.graph));
       p.graph = nullptr;
       hipGraph_t g = nullptr;
@@ -872,6 +903,7 @@ static void enqueue_plan(BuildPlan& p, const double* dD, double* dG, const doubl
       p.graph_layout = p.layout;
       p.graph_stream = st;
       p.graph_any_dn = p.any_dn;
+      p.graph_eps = p.eps;
     }
     enqueue_prologue(p, st);
     enqueue_classes(p, st, false, 2, false);
@@ -883,7 +915,7 @@ static void enqueue_plan(BuildPlan& p, const double* dD, double* dG, const doubl
     enqueue_classes(p, st, timed, 0);
     enqueue_epilogue(p, st);
   }
-  HIPCHK(hipEventRecord(c.ev1, st));
+  if (record) HIPCHK(hipEventRecord(c.ev1, st));
 }
 
 // ------------------------------------------------ build without allocation
@@ -965,6 +997,7 @@ static void enqueue_noalloc(BuildPlan& p, Species& s, const double* dD, double* 
   Y.Graw = Graw;
   Y.G = dG;
   Y.n = s.nbasis;
+  p.eps = s.screen_eps;
   hipLaunchKernelGGL(store_desc_kernel, dim3(1), dim3(1), 0, st, md, M, sd, Y);
   if (M.screen || M.mixed) launch_dnorm_batch(md, 1, M.nshell, st);
   launch_fill_zero(Graw, (size_t)M.n * M.n, st);
@@ -1008,6 +1041,7 @@ static void compute_2body(Species& s, const double* dens, double* result, double
   ensure_context();
   hipStream_t st = c.stream;
   const size_t nn = (size_t)s.nbasis * s.nbasis;
+  if (s.setup_each_build) s.sig.valid = false;
   BuildPlan& p = own_plan(s);
   s.d_D.upload(dens, nn, st);
   s.d_G.ensure(nn);
@@ -1022,16 +1056,18 @@ static void compute_2body(Species& s, const double* dens, double* result, double
 // pointers; either result may be null).  Results are zeroed here by a fill
 // kernel; the classes run on the heavy-kernel stream, joined back to st.
 static void enqueue_coupling(Species& a, Species& b, const double* dDb, double* dRa, const double* dDa, double* dRb,
-                             hipStream_t st, lowdin_fock_stats* stt) {
+                             hipStream_t st, lowdin_fock_stats* stt, bool inline_launch = false) {
   Context& c = ctx();
   ensure_all(a, st);
   ensure_all(b, st);
   TaskSet* ts = coupling_taskset(a, b, st);
   if (dRa) launch_fill_zero(dRa, (size_t)a.nbasis * a.nbasis, st);
   if (dRb) launch_fill_zero(dRb, (size_t)b.nbasis * b.nbasis, st);
-  hipStream_t hs = c.side[0];
-  HIPCHK(hipEventRecord(c.fork, st));
-  HIPCHK(hipStreamWaitEvent(hs, c.fork, 0));
+  hipStream_t hs = inline_launch ? st : c.side[0];
+  if (!inline_launch) {
+    HIPCHK(hipEventRecord(c.fork, st));
+    HIPCHK(hipStreamWaitEvent(hs, c.fork, 0));
+  }
   uint64_t quartets = 0, eris = 0, prims = 0;
   for (const TaskSet::Launch& L : ts->launches) {
     const bool a_bra = L.bra_is_a != 0;
@@ -1057,7 +1093,11 @@ static void enqueue_coupling(Species& a, Species& b, const double* dDb, double* 
       g.rbra = dRb;  g.dket = dDa;
       g.rket = dRa;  g.dbra = dDb;
     }
-    if (!dispatch_cp(L.la, L.lb, L.lc, L.ld, g, hs)) throw std::runtime_error("no coupling kernel for class");
+    // from total L = kCoopDefaultL up one wavefront per quartet (no private
+    // segment); below, one lane per quartet
+    const bool coop = L.la + L.lb + L.lc + L.ld >= kCoopDefaultL && has_jk_coop(L.la, L.lb, L.lc, L.ld);
+    if (!(coop ? dispatch_cp_coop(L.la, L.lb, L.lc, L.ld, g, hs) : dispatch_cp(L.la, L.lb, L.lc, L.ld, g, hs)))
+      throw std::runtime_error("no coupling kernel for class");
     if (stt) {
       for (size_t k = L.off; k < L.off + L.count; ++k) {
         const Task& t = ts->tasks[k];
@@ -1070,8 +1110,10 @@ static void enqueue_coupling(Species& a, Species& b, const double* dDb, double* 
     }
   }
   HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(c.join[0], hs));
-  HIPCHK(hipStreamWaitEvent(st, c.join[0], 0));
+  if (!inline_launch) {
+    HIPCHK(hipEventRecord(c.join[0], hs));
+    HIPCHK(hipStreamWaitEvent(st, c.join[0], 0));
+  }
   if (stt) {
     stt->quartets = quartets;
     stt->eris = eris;
@@ -1810,6 +1852,7 @@ int lowdin_fock_compute_2body_device(void* lint, const double* d_dens, double* d
     const double t0 = lowdin::now_ms();
     lowdin::ensure_context();
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c.stream;
+    if (s->setup_each_build) s->sig.valid = false;
     lowdin::BuildPlan& p = lowdin::own_plan(*s);
     lowdin::enqueue_plan(p, d_dens, d_result, &factor, st);
     if (synchronize) {
@@ -1868,6 +1911,22 @@ int lowdin_fock_set_precision(void* lint, int mode, double tau) {
   // the pair order depends on the mode: re-sort on the next build
   if (s->sig.valid && s->sig_mixed != mode) s->sig.valid = false;
   lowdin::g_last_error.clear();
+  return 0;
+}
+
+int lowdin_fock_set_screen(void* lint, double eps) {
+  Species* s = static_cast<Species*>(lint);
+  if (!(eps >= 0.0)) {
+    lowdin::g_last_error = "set_screen: the threshold must be >= 0";
+    return -1;
+  }
+  s->screen_eps = eps;
+  lowdin::g_last_error.clear();
+  return 0;
+}
+
+int lowdin_fock_set_setup_each_build(void* lint, int enable) {
+  static_cast<Species*>(lint)->setup_each_build = enable != 0;
   return 0;
 }
 
@@ -2192,3 +2251,30 @@ int lowdin_fock_batch_targets(void* batch, int* targets) {
 }
 
 }  // extern "C"
+
+namespace lowdin {
+// Capturable enqueue paths for the batched SCF (lowdin_scf_batch.cpp): the
+// batch's class launches on per-molecule device buffers, and one fused
+// coupling pass; no waits, no events of their own, no statistics.  The
+// first call after a change of the molecules allocates and must not be
+// under graph capture.
+void internal_enqueue_batch_build(void* batch, const double* const* dD, double* const* dG, const double* factors,
+                                  hipStream_t st, bool inline_launch) {
+  Context& c = ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  ensure_context();
+  BuildPlan& p = static_cast<lowdin_fock_batch_t*>(batch)->plan;
+  p.inline_launch = inline_launch;
+  if (p.use_graph || p.time_launches) throw std::runtime_error("batched SCF builds run without graph or timing");
+  enqueue_plan(p, nullptr, nullptr, factors, st, dD, dG, false);
+}
+
+void internal_enqueue_coupling_pair(void* a, void* b, const double* dDa, const double* dDb, double* dRa, double* dRb,
+                                    hipStream_t st, bool inline_launch) {
+  Context& c = ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  ensure_context();
+  enqueue_coupling(*static_cast<Species*>(a), *static_cast<Species*>(b), dDb, dRa, dDa, dRb, st, nullptr,
+                   inline_launch);
+}
+}  // namespace lowdin

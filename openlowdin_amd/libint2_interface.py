@@ -93,6 +93,10 @@ def lib():
         L.lowdin_fock_compute_2body_device.restype = C.c_int
         L.lowdin_fock_compute_2body_device_noalloc.argtypes = [vp, vp, vp, C.c_double, vp]
         L.lowdin_fock_compute_2body_device_noalloc.restype = C.c_int
+        L.lowdin_fock_set_screen.argtypes = [vp, C.c_double]
+        L.lowdin_fock_set_screen.restype = C.c_int
+        L.lowdin_fock_set_setup_each_build.argtypes = [vp, C.c_int]
+        L.lowdin_fock_set_setup_each_build.restype = C.c_int
         L.lowdin_fock_time_launches.argtypes = [vp, C.c_int]
         L.lowdin_fock_serialize_launches.argtypes = [vp, C.c_int]
         L.lowdin_fock_1body.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp]
@@ -413,6 +417,27 @@ class Libint2Interface:
         m = {"fp64": 0, "mixed": 1}[mode]
         if lib().lowdin_fock_set_precision(self.this, m, float(tau)) != 0:
             _check("set_precision")
+
+    def set_screen(self, eps: float):
+        """Quartet screen threshold (default DBL_EPSILON, the reference's)."""
+        if lib().lowdin_fock_set_screen(self.this, float(eps)) != 0:
+            _check("set_screen")
+
+    def set_setup_each_build(self, enable: bool = True):
+        """Recompute pair records and Schwarz bounds before every build (the
+        reference's per-call setup)."""
+        lib().lowdin_fock_set_setup_each_build(self.this, 1 if enable else 0)
+
+    def coupling_pair_device(self, other: "Libint2Interface", d_self: int, d_other: int, d_rself: int,
+                             d_rother: int, stream: int = 0, synchronize: bool = True):
+        """Both coupling directions in one pass on device buffers: d_rself =
+        coupling on this species from d_other, d_rother = on the other from
+        d_self (row-major images of the reference's results)."""
+        rc = lib().lowdin_fock_coupling_pair_device(self.this, other.this, d_self, d_other, d_rself, d_rother,
+                                                    stream or None, 1 if synchronize else 0)
+        if rc != 0:
+            _check("coupling_pair_device")
+            raise LowdinFockError("coupling_pair_device failed")
 
     def serialize_launches(self, enable: bool = True):
         lib().lowdin_fock_serialize_launches(self.this, 1 if enable else 0)

@@ -1,6 +1,6 @@
 """The wave-cooperative programs of the high-L classes (gen_eri.py, CoopGen)
-executed on the CPU exactly as a wavefront runs them -- per round every lane
-reads its LDS operands, then every lane writes -- and compared with the
+executed on the CPU exactly as a wavefront runs them -- per group of
+independent rounds every lane reads its LDS operands, then every lane writes -- and compared with the
 oracle's primitive ERIs (McMurchie-Davidson, pinned against the reference's
 own Rys code), single primitive quartets and a contracted two-primitive sum.
 
@@ -29,47 +29,61 @@ def _decode(words, r):
 
 def run_program(gen, prims, AB, CD):
     """prims: list of dicts with PA, WP, QC, WQ (3-vectors), o2p, o2q, o2pq,
-    rop, roq, F (scaled Boys values F_0..F_L)."""
+    rop, roq, F (scaled Boys values F_0..F_L).  Executes the rounds in the
+    kernel's groups: every lane reads the operands of all rounds of a group,
+    then every lane writes."""
     vrounds, hrounds, words, out0 = gen.build()
+    vg, ag, hg = gen.groups
+    assert sum(n for _, n in vg) == len(vrounds) and sum(n for _, n in hg) == len(hrounds)
     info = gen.info
     R = np.full(info["region"] + 8, np.nan)
     R[G.COOP_ZERO] = R[G.COOP_ZERO + 1] = 0.0
     R[G.COOP_ACC:G.COOP_ACC + gen.nacc] = 0.0
     nv, na = len(vrounds), info["acc_rounds"]
+
+    def vrr_op(x, y, kind, pq):
+        s0, k1 = (x & 0xfff8) // 8, x & 7
+        s2, k2 = ((x >> 16) & 0xfff8) // 8, (x >> 16) & 7
+        s4 = (y & 0xfff8) // 8
+        dst, i = ((y >> 16) & 0xfff8) // 8, (y >> 16) & 3
+        c1, c2 = R[6 * kind + 2 * i], R[6 * kind + 2 * i + 1]
+        if kind:
+            v = c1 * R[s0] + c2 * R[s0 + 1] + k1 * pq["o2q"] * (R[s2] - pq["roq"] * R[s2 + 1]) \
+                + k2 * pq["o2pq"] * R[s4]
+        else:
+            v = c1 * R[s0] + c2 * R[s0 + 1] + k1 * pq["o2p"] * (R[s2] - pq["rop"] * R[s2 + 1])
+        return dst, v
+
     for pq in prims:
         R[0:6] = [pq["PA"][0], pq["WP"][0], pq["PA"][1], pq["WP"][1], pq["PA"][2], pq["WP"][2]]
         R[6:12] = [pq["QC"][0], pq["WQ"][0], pq["QC"][1], pq["WQ"][1], pq["QC"][2], pq["WQ"][2]]
         R[info["base"]:info["base"] + gen.L + 1] = pq["F"]
-        for r, (kind, _) in enumerate(vrounds):
-            vals = []
-            for x, y in _decode(words, r):
-                s0, k1 = (x & 0xfff8) // 8, x & 7
-                s2, k2 = ((x >> 16) & 0xfff8) // 8, (x >> 16) & 7
-                s4 = (y & 0xfff8) // 8
-                dst, i = ((y >> 16) & 0xfff8) // 8, (y >> 16) & 3
-                c1, c2 = R[6 * kind + 2 * i], R[6 * kind + 2 * i + 1]
-                if kind:
-                    v = c1 * R[s0] + c2 * R[s0 + 1] + k1 * pq["o2q"] * (R[s2] - pq["roq"] * R[s2 + 1]) \
-                        + k2 * pq["o2pq"] * R[s4]
-                else:
-                    v = c1 * R[s0] + c2 * R[s0 + 1] + k1 * pq["o2p"] * (R[s2] - pq["rop"] * R[s2 + 1])
-                vals.append((dst, v))
+        r = 0
+        for kind, nr in vg:
+            assert all(vrounds[r + k][0] == kind for k in range(nr))
+            vals = [vrr_op(x, y, kind, pq) for k in range(nr) for x, y in _decode(words, r + k)]
             for dst, v in vals:
                 R[dst] = v
-        for r in range(na):
-            vals = [(y // 8, R[y // 8] + R[x // 8]) for x, y in _decode(words, nv + r)]
+            r += nr
+        r = nv
+        for _, nr in ag:
+            vals = [(y // 8, R[y // 8] + R[x // 8]) for k in range(nr) for x, y in _decode(words, r + k)]
             for dst, v in vals:
                 R[dst] = v
+            r += nr
     R[12:15] = AB
     R[15:18] = CD
-    for r, (kind, _) in enumerate(hrounds):
+    r = nv + na
+    for kind, nr in hg:
         vals = []
-        for x, y in _decode(words, nv + na + r):
-            xs, ys = (x & 0xffff) // 8, (x >> 16) // 8
-            dst, i = (y & 0xfff8) // 8, y & 3
-            vals.append((dst, R[xs] + R[(15 if kind else 12) + i] * R[ys]))
+        for k in range(nr):
+            for x, y in _decode(words, r + k):
+                xs, ys = (x & 0xffff) // 8, (x >> 16) // 8
+                dst, i = (y & 0xfff8) // 8, y & 3
+                vals.append((dst, R[xs] + R[(15 if kind else 12) + i] * R[ys]))
         for dst, v in vals:
             R[dst] = v
+        r += nr
     return R[out0:out0 + gen.nout].copy()
 
 
