@@ -129,6 +129,7 @@ def batch_bench(steps, H, B, S, mixed_tau, stream):
     Ds = [synthetic_density(sp.nbasis, sp.n_occupied, seed=int(rng.integers(1 << 30))) for sp in sps]
     out = {"molecules": len(sps), "n_range": [min(sp.nbasis for sp in sps), max(sp.nbasis for sp in sps)],
            "density": "2 C C^T of random orbitals per molecule"}
+    G64 = None
     for mode in ("fp64", "mixed"):
         fb = B.FockBatch.from_species(sps)
         if mode == "mixed":
@@ -153,8 +154,17 @@ def batch_bench(steps, H, B, S, mixed_tau, stream):
                      "molecule_builds_per_s": round(len(sps) / sb, 1), "eris_per_s": round(st["eris"] / sb, 1),
                      "eris_per_batch": st["eris"], "fp32_eri_share": round(st["eris_fp32"] / max(st["eris"], 1), 4),
                      "autotune_s": round(t_tune, 2), "targets": fb.targets()}
-        if mode == "mixed":
+        Gs = fb.unpack(dG.download())
+        if mode == "fp64":
+            G64 = Gs
+        else:
             out[mode]["tau"] = mixed_tau
+            # every molecule's result against its FP64 build: max |dG| and
+            # the energy change at fixed density, 0.5 tr(D dG)
+            dG_max = [float(np.max(np.abs(g - g0))) for g, g0 in zip(Gs, G64)]
+            dE = [0.5 * float(np.sum(D * (g - g0))) for D, g, g0 in zip(Ds, Gs, G64)]
+            out[mode]["max_abs_dG_vs_fp64"] = max(dG_max)
+            out[mode]["max_abs_dE_fixed_density_vs_fp64"] = max(abs(x) for x in dE)
         fb.close()
     # the same molecules one build per call (device-resident, FP64)
     hs = [LI.Libint2Interface(sp, 1) for sp in sps]
@@ -186,11 +196,13 @@ def batched_scf_bench(S, H, mixed_tau):
     from openlowdin_amd import scf_batch as SB
     systems = S.water_cluster_batch(256)
     out = {"molecules": len(systems)}
+    energies = {}
     for mode, tau in (("fp64", None), ("mixed", mixed_tau)):
         eng = SB.BatchSCF(use_graph=True)
         jobs = [eng.add(s, mixed_tau=tau) for s in systems]
         live = dict(enumerate(jobs))
         prev, its, conv = {}, {}, 0
+        energies[mode] = {}
         step_ms = []
         t0 = time.perf_counter()
         while live:
@@ -204,6 +216,7 @@ def batched_scf_bench(S, H, mixed_tau):
                 if code != 0 or ok or it >= 200:
                     conv += int(ok)
                     its[idx] = it
+                    energies[mode][idx] = e
                     eng.remove(j)
                     del live[idx]
         dt = time.perf_counter() - t0
@@ -215,6 +228,9 @@ def batched_scf_bench(S, H, mixed_tau):
                      "step_ms_all_256_active": round(float(np.median(full)), 3)}
         if tau is not None:
             out[mode]["tau"] = tau
+            dE = [energies[mode][k] - energies["fp64"][k] for k in energies[mode]]
+            out[mode]["max_abs_dE_vs_fp64"] = max(abs(x) for x in dE)
+            out[mode]["mean_abs_dE_vs_fp64"] = float(np.mean(np.abs(dE)))
         print("[bench] batched SCF %s: 256 molecules in %.2f s (%d steps, %.1f ms per step with all active)" % (
             mode, dt, len(step_ms), out[mode]["step_ms_all_256_active"]), file=sys.stderr, flush=True)
     # one batched build of the same 256 molecules (graph replay) for comparison
@@ -248,12 +264,14 @@ def serving_bench(S, SV, njobs, rel_rates, mixed_tau, seed=20261202):
     rng = np.random.default_rng(seed)
     jobs = [S.serving_job(rng) for _ in range(njobs)]
 
-    def run(arrivals, **kw):
+    def run(arrivals, energies=None, **kw):
         eng = SV.ScfEngine(hbm_budget=200e9, **kw)
         for k, ((label, sysm), t) in enumerate(zip(jobs, arrivals)):
             eng.submit(SV.Job(k, label, sysm, float(t)))
         eng.run(timeout=900)
         rep = eng.report()
+        if energies is not None:
+            energies.update({j.jid: j.energy for j in eng.done if j.converged})
         eng.close()
         return rep
 
@@ -270,8 +288,14 @@ def serving_bench(S, SV, njobs, rel_rates, mixed_tau, seed=20261202):
         lam = rel * lam_sat
         arrivals = np.cumsum(gaps) / lam
         row = {"rate_rel_saturation": rel, "lambda_jobs_per_s": round(lam, 4)}
-        for mode, tau in (("fp64", None), ("mixed", mixed_tau)):
-            row[mode] = run(arrivals, mixed_tau=tau)
+        e64, emx = {}, {}
+        for mode, tau, en in (("fp64", None, e64), ("mixed", mixed_tau, emx)):
+            row[mode] = run(arrivals, energies=en, mixed_tau=tau)
+        # every job's mixed-precision result against its FP64 result
+        dE = [emx[k] - e64[k] for k in emx if k in e64]
+        if dE:
+            row["mixed"]["max_abs_dE_vs_fp64"] = max(abs(x) for x in dE)
+            row["mixed"]["jobs_compared"] = len(dE)
         out["rates"].append(row)
         print("[bench] serving at %.2fx: fp64 p50 %.2f s p99 %.2f s; mixed p50 %.2f s p99 %.2f s" % (
             rel, row["fp64"]["p50_s"], row["fp64"]["p99_s"], row["mixed"]["p50_s"], row["mixed"]["p99_s"]),
@@ -542,10 +566,13 @@ def main():
     ap.add_argument("--taus", default="1e-2,1e-3,1e-4,1e-5,1e-6,1e-7")
     ap.add_argument("--no-batch-scf", action="store_true", help="skip the 256-cluster batched SCF")
     ap.add_argument("--no-serving", action="store_true", help="skip the served-SCF runs")
-    ap.add_argument("--serving-jobs", type=int, default=16)
-    ap.add_argument("--serving-rates", default="0.5,1,2")
+    ap.add_argument("--serving-jobs", type=int, default=96)
+    ap.add_argument("--serving-rates", default="1")
     ap.add_argument("--no-store", action="store_true", help="skip the integral store / transform runs")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default per-class kernel shapes")
+    ap.add_argument("--main-tau", type=float, default=None,
+                    help="run the timed C20H42 builds in mixed precision with this tau (profiling; the headline "
+                         "line is FP64)")
     ap.add_argument("--no-g12", action="store_true", help="skip the Gaussian-geminal builds")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the per-call-setup and 1e-12-screen variants of the headline build")
@@ -575,6 +602,8 @@ def main():
     D = synthetic_density(n, sp.n_occupied)
     h = LI.Libint2Interface(sp, 1)
     h.init_2body_ints()
+    if args.main_tau is not None:
+        h.set_precision("mixed", args.main_tau)
     stream = H.Stream()
     dD = H.DeviceArray.from_host(np.asfortranarray(D).ravel(order="F"))
     dG = H.DeviceArray((n * n,))
@@ -660,18 +689,21 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.main_tau is None else "f64+f32",
         "data": "synthetic",
         "config": {"workload": "C20H42 cc-pVDZ direct RHF Fock build (J - 0.5 K), n=510 Cartesian, "
                                "pair list 1e-12, Schwarz x density screen at DBL_EPSILON",
-                   "precision": "fp64", "density": "2 C C^T, 81 random orbitals (seed 17)"},
+                   "precision": "fp64" if args.main_tau is None else "mixed, tau %g" % args.main_tau,
+                   "density": "2 C C^T, 81 random orbitals (seed 17)"},
         "eris_per_s": round(st["eris"] * builds_per_s, 1),
         "eris_per_build": st["eris"],
         "quartets_per_build": st["quartets"],
         "prim_quartets_per_build": st["prim_quartets"],
         "event_ms_per_step": round(ev_ms / args.steps, 4),
         "build_tflops_fp64": round(total_flops * builds_per_s / 1e12, 3),
-        "roofline": {"bound": "valu", "kernel": kname,
+        "roofline": {"bound": "mfma", "pipe": "FP64 VALU: the recursions are not matrix products; 78.6 TFLOP/s is "
+                                              "MI355X's FP64 peak for both the matrix and the vector units",
+                     "kernel": kname,
                      "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic, "traffic_unit": "bytes/launch",
                      "traffic_source": "profiles/jk_c20h42_fp64_traffic.md" if traffic else None, "flops_per_launch": fl,

@@ -43,7 +43,7 @@ __device__ __forceinline__ double& lds_ref(double* R, unsigned byteoff) {
 }
 
 // Independent rounds a lane takes at once (COOP_GROUP of gen_eri.py).
-constexpr int kCoopGroup = 8;
+constexpr int kCoopGroup = 4;
 
 // Group g of a stage: 4 bits (rounds - 1, kind << 3) packed 16 per word.
 template <unsigned long long G0, unsigned long long G1>

@@ -516,18 +516,19 @@ LOWDIN_STRIP_WAVES(2, 2, 0, 0, 2)
 // in row b), so those are accumulated with LDS atomics in two row strips
 // (NA + NB) x n held for the whole ket stream and flushed to HBM once, with
 // one global atomic per touched element.  J_cd, distinct for every ket, is
-// the only per-quartet global atomic.  The bra's FP64 primitive-pair
-// records (at most kMaxStripPrims, guaranteed by the host) are staged in
-// LDS once: every lane reads the same record, an LDS broadcast.  The FP32
-// path of mixed precision reads its records through generic pointers: read
-// through global-address-space views its integrals came out wrong on the
-// GPU (cause not found; the FP64 path is unaffected).
+// the only per-quartet global atomic.  The bra's primitive-pair records
+// (at most kMaxStripPrims, guaranteed by the host; FP64, and in mixed mode
+// also the FP32 copies) are staged in LDS once: every lane reads the same
+// record, an LDS broadcast.  The FP32 ket records are read through generic
+// pointers: read through global-address-space views the FP32 integrals
+// once came out wrong on the GPU (the FP64 path is unaffected).
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(256, (StripWaves<LA, LB, LC, LD>::value)) void jk_strip_kernel(JKArgs a) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr bool kF32 = Fp32Allowed<LA, LB, LC, LD>::value;
   extern __shared__ double strip[];  // rows of a (NA x n), then rows of b (NB x n)
   __shared__ double bpd[kMaxStripPrims * kPrimPairStride];
+  __shared__ float bpf[kF32 ? kMaxStripPrims * kPrimPairStride : 1];   // FP32 bra records (mixed)
   __shared__ unsigned int wg_q, wg_p, wg_q32;
   const int tid = threadIdx.x;
   const Task t = a.tasks[blockIdx.x];
@@ -545,6 +546,12 @@ __global__ __launch_bounds__(256, (StripWaves<LA, LB, LC, LD>::value)) void jk_s
   {
     const gcptr<double> src = prim + (size_t)P.poff * kPrimPairStride;
     for (int k = tid; k < P.nprim * kPrimPairStride; k += blockDim.x) bpd[k] = src[k];
+    if constexpr (kF32) {
+      if (M.mixed) {
+        const float* srcf = M.primf + (size_t)P.poff * kPrimPairStride;
+        for (int k = tid; k < P.nprim * kPrimPairStride; k += blockDim.x) bpf[k] = srcf[k];
+      }
+    }
   }
   if (tid == 0) {
     wg_q = 0;
@@ -582,7 +589,7 @@ __global__ __launch_bounds__(256, (StripWaves<LA, LB, LC, LD>::value)) void jk_s
     if constexpr (kF32) {
       if (lo) {
         myq32 += 1;
-        contracted_quartet_f32<LA, LB, LC, LD>(M.primf + (size_t)P.poff * kPrimPairStride, P.nprim,
+        contracted_quartet_f32<LA, LB, LC, LD>((const float*)bpf, P.nprim,
                                                M.primf + (size_t)K.poff * kPrimPairStride, K.nprim, a.boysf,
                                                P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
       }

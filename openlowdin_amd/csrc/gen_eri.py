@@ -304,7 +304,7 @@ class HRRGen:
 COOP_P, COOP_ZERO, COOP_TRASH, COOP_ACC = 0, 18, 20, 22
 # Rounds of one recursion level and kind are independent: a lane loads the
 # descriptors and operands of up to COOP_GROUP of them before computing.
-COOP_GROUP = 8   # kCoopGroup of eri_coop.h
+COOP_GROUP = 4   # kCoopGroup of eri_coop.h
 
 
 def coop_groups(rounds):

@@ -503,8 +503,14 @@ static void sort_by_cost(std::vector<Task>& t, size_t from, const PairInfo* bras
 // Largest LDS row-strip footprint a J/K workgroup may use.
 constexpr size_t kStripLdsMax = 65536;
 // Classes from this total angular momentum up use the wave-cooperative
-// kernel unless the autotuner chose otherwise.
+// kernel unless the autotuner chose otherwise: by default without row strips
+// (exchange updates as global atomics) and cut into about kCoopDefaultWg
+// workgroups of four wavefronts, at least kCoopMinRun kets each.  A
+// wavefront evaluates one quartet at a time, so a class of 10^5 quartets
+// needs thousands of wavefronts to fill the GPU's 1024 SIMDs.
 constexpr int kCoopDefaultL = 6;
+constexpr int kCoopDefaultWg = 4096;
+constexpr long kCoopMinRun = 4;
 
 static inline int class_pair_id(int cx, int cy) { return cx * (cx + 1) / 2 + cy; }
 
@@ -569,9 +575,8 @@ static void build_plan_tasks(BuildPlan& p, hipStream_t st) {
       int kd = p.kind[class_pair_id(cx, cy)];
       // wave-cooperative kernel: by default from total L = kCoopDefaultL up
       const bool coop_ok = has_jk_coop(la, lb, lc, ld);
-      const int coop = !coop_ok ? 0
-                       : (kd == 4 || (kd == 0 && la + lb + lc + ld >= kCoopDefaultL)) ? 3
-                       : kd == 5 ? 4 : 0;
+      const bool coop_default = kd == 0 && la + lb + lc + ld >= kCoopDefaultL;
+      const int coop = !coop_ok ? 0 : kd == 4 ? 3 : (kd == 5 || coop_default) ? 4 : 0;
       if (kd >= 4) kd = 0;
       const bool strip_ok = (size_t)(ncart_h(la) + ncart_h(lb)) * p.nmax * sizeof(double) <= kStripLdsMax &&
                             maxbra <= kMaxStripPrims;
@@ -583,9 +588,10 @@ static void build_plan_tasks(BuildPlan& p, hipStream_t st) {
       const size_t from = ts.tasks.size();
       // run length: at least enough kets to amortise zeroing and flushing the
       // strips (which grow with n), at most 32768
-      const long cmin = std::max(64L, std::min(256L, (long)((p.nmax / 2 + 63) / 64) * 64));
-      long chunk = (total + p.target_wg[class_pair_id(cx, cy)] - 1) / p.target_wg[class_pair_id(cx, cy)];
-      chunk = std::max(cmin, std::min(32768L, ((chunk + 63) / 64) * 64));
+      const long cmin = coop ? kCoopMinRun : std::max(64L, std::min(256L, (long)((p.nmax / 2 + 63) / 64) * 64));
+      const long tgt = coop_default ? kCoopDefaultWg : p.target_wg[class_pair_id(cx, cy)];
+      long chunk = (total + tgt - 1) / tgt;
+      chunk = std::max(cmin, std::min(32768L, coop ? ((chunk + 3) / 4) * 4 : ((chunk + 63) / 64) * 64));
       for (int m = 0; m < nm; ++m) {
         const PairSet& ps = p.mols[m]->sig;
         const int nx = ps.off[cx + 1] - ps.off[cx], ny = ps.off[cy + 1] - ps.off[cy];
@@ -2135,8 +2141,8 @@ static int plan_autotune(lowdin::BuildPlan& p, const double* d_dens, double* d_r
   hipStream_t st = stream ? stream : c.stream;
   // candidates: the wavefront kernel, LDS strips aiming at that many
   // workgroups per launch, and LDS strips with one-wavefront workgroups
-  static const int cand[][2] = {{2, 2048}, {1, 512}, {1, 2048}, {1, 8192}, {3, 512},
-                                {4, 512}, {4, 2048}, {5, 512}, {5, 2048}};   // {kind, workgroups}
+  static const int cand[][2] = {{2, 2048}, {1, 512}, {1, 2048}, {1, 8192}, {3, 512}, {4, 2048},
+                                {4, 8192}, {5, 2048}, {5, 8192}, {5, 32768}};   // {kind, workgroups}
   const bool tl = p.time_launches, se = p.serialize, ug = p.use_graph;
   p.time_launches = true;
   p.serialize = true;
