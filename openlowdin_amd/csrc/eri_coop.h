@@ -68,6 +68,35 @@ __host__ __device__ constexpr int coop_wave_doubles() {
   return (CoopProgram<LA, LB, LC, LD>::kRegion + NA * NB + NC * ND + NA * NC + NB * ND + NA * ND + NB * NC + 1) & ~1;
 }
 
+// F_m(T) for one order m per lane (the cooperative kernels need F_0..F_L
+// on lanes 0..L): below kBoysTmax the seventh-order Taylor expansion of F_m
+// itself around the nearest tabulated point (columns 1 + m .. 8 + m of the
+// row), F_m(T) = sum_k F_{m+k}(T_i) (T_i - T)^k / k!; above, the asymptotic
+// F_0 = sqrt(pi / T) / 2 and the upward recursion
+// F_{k+1} = ((2k + 1) F_k - e^-T) / (2T) up to m.  No array of orders per
+// lane, no downward recursion.
+__device__ __forceinline__ double boys_one(double T, int m, const double* __restrict__ tab) {
+  if (T < kBoysTmax) {
+    const int i = (int)(T * kBoysInvH + 0.5);
+    const double x = (double)i * kBoysH - T;
+    const double* r = tab + i * kBoysCols + 1 + m;
+    double s = r[7];
+    s = fma(s, x * (1.0 / 7.0), r[6]);
+    s = fma(s, x * (1.0 / 6.0), r[5]);
+    s = fma(s, x * (1.0 / 5.0), r[4]);
+    s = fma(s, x * (1.0 / 4.0), r[3]);
+    s = fma(s, x * (1.0 / 3.0), r[2]);
+    s = fma(s, x * (1.0 / 2.0), r[1]);
+    return fma(s, x, r[0]);
+  }
+  const double rt = rsqrt_nr(T);
+  const double o2t = 0.5 * rt * rt;
+  const double eT = exp(-T);
+  double f = 0.88622692545275801365 * rt;
+  for (int k = 0; k < m; ++k) f = ((double)(2 * k + 1) * f - eT) * o2t;
+  return f;
+}
+
 // One contracted quartet of class (LA LB|LC LD), raw (unnormalised
 // functions), by one wavefront: the [e0|f0] accumulators are cleared, every
 // primitive quartet (bra primitive pairs bprim[0..nbp), ket kprim[0..nkp))
@@ -95,8 +124,8 @@ __device__ __forceinline__ void coop_quartet(double* R, const uint2* pg, gcptr<d
       const double rop = q * inv, roq = p * inv;
       const double T = p * rop * (PQx * PQx + PQy * PQy + PQz * PQz);
       const double pref = bp[8] * Kcd * rs;
-      double F[L + 1];
-      boys_f64<L>(T, boys, F);
+      // lane m <= L: the scaled Boys value F_m(T) * pref
+      if (lane <= L) R[Pg::kBase + lane] = boys_one(T, lane, boys) * pref;
       if (lane == 0) {
         R[0] = bp[5];
         R[1] = -rop * PQx;
@@ -111,9 +140,6 @@ __device__ __forceinline__ void coop_quartet(double* R, const uint2* pg, gcptr<d
         R[10] = kq[7];
         R[11] = roq * PQz;
       }
-#pragma unroll
-      for (int m = 0; m <= L; ++m)
-        if (lane == m) R[Pg::kBase + m] = F[m] * pref;
       const double o2pq = 0.5 * inv;
       // VRR: groups of independent rounds; a lane loads every descriptor
       // and operand of a group before it writes, and the next group's

@@ -502,12 +502,13 @@ static void sort_by_cost(std::vector<Task>& t, size_t from, const PairInfo* bras
 
 // Largest LDS row-strip footprint a J/K workgroup may use.
 constexpr size_t kStripLdsMax = 65536;
-// Classes from this total angular momentum up use the wave-cooperative
-// kernel unless the autotuner chose otherwise: by default without row strips
-// (exchange updates as global atomics) and cut into about kCoopDefaultWg
-// workgroups of four wavefronts, at least kCoopMinRun kets each.  A
-// wavefront evaluates one quartet at a time, so a class of 10^5 quartets
-// needs thousands of wavefronts to fill the GPU's 1024 SIMDs.
+// Classes from this total angular momentum up run only the wave-cooperative
+// kernel (the per-lane kernels would spill kilobytes per lane to scratch):
+// by default without row strips (exchange updates as global atomics) and
+// cut into about kCoopDefaultWg workgroups of four wavefronts, at least
+// kCoopMinRun kets each; the autotuner may choose strips and another
+// workgroup count.  A wavefront evaluates one quartet at a time, so a class
+// of 10^5 quartets needs thousands of wavefronts to fill the 1024 SIMDs.
 constexpr int kCoopDefaultL = 6;
 constexpr int kCoopDefaultWg = 4096;
 constexpr long kCoopMinRun = 4;
@@ -575,7 +576,7 @@ static void build_plan_tasks(BuildPlan& p, hipStream_t st) {
       int kd = p.kind[class_pair_id(cx, cy)];
       // wave-cooperative kernel: by default from total L = kCoopDefaultL up
       const bool coop_ok = has_jk_coop(la, lb, lc, ld);
-      const bool coop_default = kd == 0 && la + lb + lc + ld >= kCoopDefaultL;
+      const bool coop_default = kd < 4 && la + lb + lc + ld >= kCoopDefaultL;
       const int coop = !coop_ok ? 0 : kd == 4 ? 3 : (kd == 5 || coop_default) ? 4 : 0;
       if (kd >= 4) kd = 0;
       const bool strip_ok = (size_t)(ncart_h(la) + ncart_h(lb)) * p.nmax * sizeof(double) <= kStripLdsMax &&
