@@ -700,7 +700,9 @@ def main():
         "config": {"workload": "C20H42 cc-pVDZ direct RHF Fock build (J - 0.5 K), n=510 Cartesian, "
                                "pair list 1e-12, Schwarz x density screen at DBL_EPSILON",
                    "precision": "fp64" if args.main_tau is None else "mixed, tau %g" % args.main_tau,
-                   "density": "2 C C^T, 81 random orbitals (seed 17)"},
+                   "density": "2 C C^T, 81 random orbitals (seed 17)",
+                   "setup": "pair list, Schwarz bounds and task lists kept while the geometry is unchanged "
+                            "(the reference recomputes them per call: variants.as_reference_setup)"},
         "eris_per_s": round(st["eris"] * builds_per_s, 1),
         "eris_per_build": st["eris"],
         "quartets_per_build": st["quartets"],
