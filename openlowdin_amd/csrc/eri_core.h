@@ -710,7 +710,10 @@ __global__ __launch_bounds__(256, (StripWaves<LA, LB, LC, LD>::value)) void jk_s
 // species' density (rket), each with the reference's element sets and drop.
 // Drop, density weights and scatter of one coupling quartet's raw block
 // (shared by the Coulomb and the Gaussian-geminal coupling kernels).
-template <int LA, int LB, int LC, int LD>
+// kWaveBra: every lane of the wavefront calls this with the same bra pair
+// (lanes without a ket pass a zero block), and the bra-side sums are reduced
+// over the wavefront before one atomic per element.
+template <int LA, int LB, int LC, int LD, bool kWaveBra = false>
 __device__ __forceinline__ void coupling_scatter(const CouplingArgs& a, const PairInfo& P, const PairInfo& K,
                                                  double* out) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
@@ -743,7 +746,13 @@ __device__ __forceinline__ void coupling_scatter(const CouplingArgs& a, const Pa
         for (int k = 0; k < NC * ND; ++k) s += w[k] * out[(ia * NB + ib) * NC * ND + k];
         const int fa = P.bfa + ia, fb = P.bfb + ib;
         const int r = fa > fb ? fa : fb, c = fa > fb ? fb : fa;
-        unsafeAtomicAdd(&a.rbra[(size_t)r * a.nbra + c], s * a.nbra_norma[fa] * a.nbra_norma[fb]);
+        if constexpr (kWaveBra) {
+          s = wave_sum(s);
+          if ((threadIdx.x & 63) == 0 && s != 0.0)
+            unsafeAtomicAdd(&a.rbra[(size_t)r * a.nbra + c], s * a.nbra_norma[fa] * a.nbra_norma[fb]);
+        } else {
+          unsafeAtomicAdd(&a.rbra[(size_t)r * a.nbra + c], s * a.nbra_norma[fa] * a.nbra_norma[fb]);
+        }
       }
   }
   if (a.rket) {
@@ -781,14 +790,23 @@ __global__ __launch_bounds__(256) void coupling_kernel(CouplingArgs a) {
   const Task t = a.tasks[wid];
   const int lane = threadIdx.x & 63;
   const int j = t.j0 + lane;
-  if (j >= t.j1) return;
+  // every lane stays for the wavefront's bra-side reduction; lanes without
+  // a ket contribute a zero block and no ket-side update
+  const bool active = j < t.j1;
   const PairInfo& P = a.bra[t.bra];
-  const PairInfo K = a.ket[j];
+  const PairInfo K = a.ket[active ? j : t.j0];
   double out[NA * NB * NC * ND];
-  contracted_quartet<LA, LB, LC, LD>(a.bprim + (size_t)P.poff * kPrimPairStride, P.nprim,
-                                     a.kprim + (size_t)K.poff * kPrimPairStride, K.nprim, a.boys,
-                                     P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
-  coupling_scatter<LA, LB, LC, LD>(a, P, K, out);
+  if (active) {
+    contracted_quartet<LA, LB, LC, LD>(a.bprim + (size_t)P.poff * kPrimPairStride, P.nprim,
+                                       a.kprim + (size_t)K.poff * kPrimPairStride, K.nprim, a.boys,
+                                       P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
+  } else {
+#pragma unroll
+    for (int k = 0; k < NA * NB * NC * ND; ++k) out[k] = 0.0;
+  }
+  CouplingArgs b = a;
+  if (!active) b.rket = nullptr;
+  coupling_scatter<LA, LB, LC, LD, true>(b, P, K, out);
 }
 
 // --------------------------------------------------- Schwarz (a11) kernel

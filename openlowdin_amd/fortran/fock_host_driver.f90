@@ -35,7 +35,7 @@ program fock_host_driver
   integer :: i, i2, ng, p
   logical :: have
   real(8), allocatable :: gexp(:), gcoef(:)
-  real(8), allocatable, target :: A3(:,:,:)
+  real(8), allocatable, target :: A3(:,:,:), A4(:,:,:,:)
   integer :: ghost_id
 
   call get_command_argument(1, dir)
@@ -142,6 +142,39 @@ program fock_host_driver
      end do
   end if
 
+  ! optional all.txt (orbital row p): the integral stores through the
+  ! MEMORY-path wrappers -- every species' own, and the inter-species one
+  ! from the lower species id -- and the inter-species first-quarter
+  ! transform on row p (the density as the coefficient matrix)
+  inquire(file=trim(dir)//"/all.txt", exist=have)
+  if (have) then
+     open(newunit=u, file=trim(dir)//"/all.txt", status="old", action="read")
+     read(u, *) p
+     close(u)
+     do k = 1, ns
+        allocate(A4(nbas(k), nbas(k), nbas(k), nbas(k)))
+        A4 = 0.0_8
+        call Libint2Interface_compute2BodyIntraspecies_direct_all(dens(k)%v, A4, objs(k))
+        call write_raw(trim(dir)//"/all_"//itoa(ids(k))//".bin", A4)
+        deallocate(A4)
+        do j = 1, ns
+           if (j == k) cycle
+           if (ids(k) < ids(j)) then
+              allocate(A4(nbas(j), nbas(j), nbas(k), nbas(k)))
+              A4 = 0.0_8
+              call Libint2Interface_compute2BodyInterspecies_direct_all(dens(k)%v, A4, objs(k), objs(j))
+              call write_raw(trim(dir)//"/allc_"//itoa(ids(k))//"_"//itoa(ids(j))//".bin", A4)
+              deallocate(A4)
+           end if
+           call Libint2Interface_compute2BodyInterspecies_direct_IT(dens(k)%v, dens(k)%v, A3, p, objs(k), objs(j))
+           open(newunit=u, file=trim(dir)//"/itc_"//itoa(ids(k))//"_"//itoa(ids(j))//".bin", access="stream", &
+                form="unformatted", status="replace", action="write")
+           write(u) A3
+           close(u)
+        end do
+     end do
+  end if
+
   do k = 1, ns
      call Libint2Interface_destructor(objs(k))
   end do
@@ -165,6 +198,15 @@ contains
     read(uu) m
     close(uu)
   end subroutine read_matrix
+
+  subroutine write_raw(path, a)
+    character(*), intent(in) :: path
+    real(8), intent(in) :: a(:,:,:,:)
+    integer :: uu
+    open(newunit=uu, file=path, access="stream", form="unformatted", status="replace", action="write")
+    write(uu) a
+    close(uu)
+  end subroutine write_raw
 
   subroutine write_matrix(path, m)
     character(*), intent(in) :: path

@@ -166,6 +166,17 @@ contains
     stop 1
   end subroutine LowdinFock_checkError
 
+  !> Stops (as the reference's callers stop on errors) when an array the C
+  !! side reads or writes in full holds fewer than `need` elements.
+  subroutine LowdinFock_checkSize(where, what, have, need)
+    character(*), intent(in) :: where, what
+    integer(8), intent(in) :: have, need
+    if (have >= need) return
+    write(*, '(A,I0,A,I0)') "lowdin_fock error in "//trim(where)//": "//trim(what)//" holds ", have, &
+         " elements, needs ", need
+    stop 1
+  end subroutine LowdinFock_checkSize
+
   !> One handle per species, shells in particle then contraction order with
   !! raw coefficients (Libint2Interface.f90:382-448).
   subroutine Libint2Interface_constructor(this, speciesID, isElectron, angularMoment, nprim, origins, &
@@ -246,6 +257,9 @@ contains
     integer, intent(in) :: p
     type(Libint2Interface), intent(inout) :: this
     integer(c_int) :: pp
+    call LowdinFock_checkSize("compute_2body_directIT", "density", size(density, kind=8), int(this%nbasis, 8)**2)
+    call LowdinFock_checkSize("compute_2body_directIT", "coefficients", size(coefficients, kind=8), &
+         int(this%nbasis, 8)**2)
     if (allocated(matrixA)) deallocate(matrixA)
     allocate(matrixA(this%nbasis, this%nbasis, this%nbasis))
     matrixA = 0.0_8
@@ -262,6 +276,9 @@ contains
     integer, intent(in) :: p
     type(Libint2Interface), intent(inout) :: this, other
     integer(c_int) :: pp
+    call LowdinFock_checkSize("compute_coupling_directIT", "density", size(density, kind=8), int(this%nbasis, 8)**2)
+    call LowdinFock_checkSize("compute_coupling_directIT", "coefficients", size(coefficients, kind=8), &
+         int(this%nbasis, 8)**2)
     if (allocated(coupling)) deallocate(coupling)
     allocate(coupling(other%nbasis, other%nbasis, this%nbasis))
     pp = int(p, c_int)
@@ -276,6 +293,8 @@ contains
     real(8), target, intent(in) :: density(:,:)
     real(8), target, intent(inout) :: ints(:,:,:,:)
     type(Libint2Interface), intent(inout) :: this
+    call LowdinFock_checkSize("compute_2body_directAll", "density", size(density, kind=8), int(this%nbasis, 8)**2)
+    call LowdinFock_checkSize("compute_2body_directAll", "ints", size(ints, kind=8), int(this%nbasis, 8)**4)
     call c_LibintInterface_compute2BodyDirectAll(this%this, c_loc(density(1,1)), c_loc(ints(1,1,1,1)))
     call LowdinFock_checkError("compute_2body_directAll")
   end subroutine Libint2Interface_compute2BodyIntraspecies_direct_all
@@ -285,6 +304,8 @@ contains
     real(8), target, intent(in) :: density(:,:)
     real(8), target, intent(inout) :: ints(:,:,:,:)
     type(Libint2Interface), intent(inout) :: this, other
+    call LowdinFock_checkSize("compute_coupling_directAll", "ints", size(ints, kind=8), &
+         int(this%nbasis, 8)**2 * int(other%nbasis, 8)**2)
     call c_LibintInterface_computeCouplingDirectAll(this%this, other%this, c_loc(density(1,1)), c_loc(ints(1,1,1,1)))
     call LowdinFock_checkError("compute_coupling_directAll")
   end subroutine Libint2Interface_compute2BodyInterspecies_direct_all

@@ -126,3 +126,50 @@ def test_fortran_driver_g12_and_first_quarter(tmp_path):
     it = np.fromfile(os.path.join(str(tmp_path), "it_1.bin"), dtype=np.float64).reshape(n, n, n)
     ref = O.compute_2body_directIT(be, De, De, 3)
     assert np.max(np.abs(it - ref)) < 1e-10
+
+
+@pytest.mark.gpu
+def test_fortran_driver_stores_and_inter_transform(tmp_path):
+    """The Fortran layer's MEMORY-path wrappers (intra- and inter-species
+    integral stores, Libint2Interface.f90:1065-1179) and the inter-species
+    first-quarter transform (:677-707): their arrays, ranks and by-reference
+    p against the same library through its C ABI from Python (the stores bit
+    for bit: each stored element is written once; the transform, which
+    accumulates with atomics, to summation order), and the transform against
+    the oracle."""
+    from openlowdin_amd import libint2_interface as LI
+    system = S.fhf_anion("6-31G")
+    e, p = system.species
+    rng = np.random.default_rng(31)
+    Ce = rng.normal(size=(e.nbasis, 10)) / np.sqrt(e.nbasis)
+    De = 2.0 * Ce @ Ce.T
+    cp = rng.normal(size=p.nbasis) / np.sqrt(p.nbasis)
+    Dp = np.outer(cp, cp)
+    _write_job(str(tmp_path), system, [De, Dp])
+    with open(os.path.join(str(tmp_path), "all.txt"), "w") as fh:
+        fh.write("2\n")
+    print("\n[case] fortran driver: FHF- 6-31G + proton, stores and inter-species transform p=2", flush=True)
+    r = subprocess.run([DRIVER, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    ne, npr = e.nbasis, p.nbasis
+    he, hp = LI.Libint2Interface(e, 1), LI.Libint2Interface(p, 2)
+    he.init_2body_ints()
+    hp.init_2body_ints()
+    rd = lambda name, shape: np.fromfile(os.path.join(str(tmp_path), name), dtype=np.float64).reshape(shape)
+    for h, D, n, sid in ((he, De, ne, 1), (hp, Dp, npr, 2)):
+        got = rd("all_%d.bin" % sid, (n, n, n, n))
+        ref = h.compute_2body_intraspecies_direct_all(D)
+        print("   store %d: %d nonzero, max|d| %.2e" % (sid, np.count_nonzero(got), np.max(np.abs(got - ref))))
+        assert np.count_nonzero(got) > 0 and np.array_equal(got, ref)
+    got = rd("allc_1_2.bin", (ne, ne, npr, npr))
+    ref = he.compute_2body_interspecies_direct_all(hp, De)
+    assert np.count_nonzero(got) > 0 and np.array_equal(got, ref)
+    be, bp = O.Basis(e), O.Basis(p)
+    for (h, o, D, n, no, sid, oid, bt, bo) in ((he, hp, De, ne, npr, 1, 2, be, bp), (hp, he, Dp, npr, ne, 2, 1, bp, be)):
+        got = rd("itc_%d_%d.bin" % (sid, oid), (n, no, no))
+        # the transform accumulates with atomics: equal to summation order
+        assert np.max(np.abs(got - h.compute_2body_interspecies_direct_IT(o, D, D, 2))) < 1e-13
+        ref = O.compute_coupling_directIT(bt, bo, sid, oid, D, 2)
+        print("   transform %d<-%d: max|d| vs oracle %.2e" % (sid, oid, np.max(np.abs(got - ref))))
+        assert np.max(np.abs(got - ref)) < 1e-10
