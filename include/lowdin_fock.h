@@ -319,9 +319,11 @@ int lowdin_fock_batch_launch_profile(void* batch, int cap, int* classes, float* 
  * into about {512, 2048, 8192} workgroups of 256 threads, the LDS-strip
  * kernel with one-wavefront workgroups (about 512) and, for total angular
  * momentum >= 5, the wave-cooperative kernel (one quartet per wavefront,
- * with or without row strips, about {512, 2048} workgroups), by timing `reps`
- * serialised builds per candidate on the given inputs.  Results are
- * unchanged by the choice up to FP64 summation order. */
+ * with row strips at about {2048, 8192} workgroups or without at about
+ * {2048, 8192, 32768}), by timing `reps` serialised builds per candidate on
+ * the given inputs.  Classes of total angular momentum >= 6 run only the
+ * wave-cooperative kernel (the per-lane kernels would spill to scratch).
+ * Results are unchanged by the choice up to FP64 summation order. */
 int lowdin_fock_batch_autotune(void* batch, const double* d_dens, double* d_result, const double* factors,
                                void* stream, int reps);
 
