@@ -461,8 +461,9 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
     double out[NA * NB * NC * ND];
     if constexpr (Fp32Allowed<LA, LB, LC, LD>::value) {
       if (lo)
-        contracted_quartet_f32<LA, LB, LC, LD>(M.primf + (size_t)P.poff * kPrimPairStride, P.nprim,
-                                               M.primf + (size_t)K.poff * kPrimPairStride, K.nprim, a.boysf,
+        contracted_quartet_f32<LA, LB, LC, LD>((gcptr<float>)M.primf + (size_t)P.poff * kPrimPairStride, P.nprim,
+                                               (gcptr<float>)M.primf + (size_t)K.poff * kPrimPairStride, K.nprim,
+                                               a.boysf,
                                                P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
     }
     if (!lo)
@@ -519,9 +520,7 @@ LOWDIN_STRIP_WAVES(2, 2, 0, 0, 2)
 // the only per-quartet global atomic.  The bra's primitive-pair records
 // (at most kMaxStripPrims, guaranteed by the host; FP64, and in mixed mode
 // also the FP32 copies) are staged in LDS once: every lane reads the same
-// record, an LDS broadcast.  The FP32 ket records are read through generic
-// pointers: read through global-address-space views the FP32 integrals
-// once came out wrong on the GPU (the FP64 path is unaffected).
+// record, an LDS broadcast.
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(256, (StripWaves<LA, LB, LC, LD>::value)) void jk_strip_kernel(JKArgs a) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
@@ -548,7 +547,7 @@ __global__ __launch_bounds__(256, (StripWaves<LA, LB, LC, LD>::value)) void jk_s
     for (int k = tid; k < P.nprim * kPrimPairStride; k += blockDim.x) bpd[k] = src[k];
     if constexpr (kF32) {
       if (M.mixed) {
-        const float* srcf = M.primf + (size_t)P.poff * kPrimPairStride;
+        const gcptr<float> srcf = (gcptr<float>)M.primf + (size_t)P.poff * kPrimPairStride;
         for (int k = tid; k < P.nprim * kPrimPairStride; k += blockDim.x) bpf[k] = srcf[k];
       }
     }
@@ -590,7 +589,8 @@ __global__ __launch_bounds__(256, (StripWaves<LA, LB, LC, LD>::value)) void jk_s
       if (lo) {
         myq32 += 1;
         contracted_quartet_f32<LA, LB, LC, LD>((const float*)bpf, P.nprim,
-                                               M.primf + (size_t)K.poff * kPrimPairStride, K.nprim, a.boysf,
+                                               (gcptr<float>)M.primf + (size_t)K.poff * kPrimPairStride, K.nprim,
+                                               a.boysf,
                                                P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
       }
     }
