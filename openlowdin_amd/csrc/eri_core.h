@@ -509,6 +509,30 @@ LOWDIN_STRIP_WAVES(2, 0, 2, 0, 2)
 LOWDIN_STRIP_WAVES(2, 2, 0, 0, 2)
 #undef LOWDIN_STRIP_WAVES
 
+// Classes whose strip kernel copies the nine Boys-table columns it reads
+// (e^-T and F_L .. F_{L+7} at the 401 grid points, 28.9 KB) into LDS once per
+// workgroup when that fits beside the row strips in 64 KB: the table gathers
+// then leave the texture-address unit, which these classes keep busy
+// (measured on C20H42 / cc-pVDZ, serialised: (ps|ss) 3.39 -> 3.23 ms,
+// (pp|ss) 1.16 -> 1.00, (ss|ss) 2.79 -> 2.74; (ps|ps) and (ds|ss) unchanged).
+template <int LA, int LB, int LC, int LD>
+struct StripLdsBoys {
+  static constexpr bool value = false;
+};
+template <> struct StripLdsBoys<0, 0, 0, 0> { static constexpr bool value = true; };
+template <> struct StripLdsBoys<1, 0, 0, 0> { static constexpr bool value = true; };
+template <> struct StripLdsBoys<1, 1, 0, 0> { static constexpr bool value = true; };
+constexpr int kLdsBoysCols = 9;
+constexpr size_t kLdsBoysBytes = (size_t)kBoysRows * kLdsBoysCols * sizeof(double);
+
+// Whether a strip launch of this class with row strips of width lds_n keeps
+// the Boys columns in LDS (host and device agree through this function).
+template <int LA, int LB, int LC, int LD>
+__host__ __device__ constexpr bool strip_lds_boys(int lds_n) {
+  return StripLdsBoys<LA, LB, LC, LD>::value &&
+         (size_t)(ncart(LA) + ncart(LB)) * (size_t)lds_n * sizeof(double) + kLdsBoysBytes <= 65536;
+}
+
 // ------------------------------------------------ J/K kernel, LDS strips
 // A workgroup owns one bra pair (a b) and streams up to t.j1 - t.j0 ket
 // pairs, one per thread per round (256 threads, or one wavefront of 64 for
@@ -542,6 +566,17 @@ __global__ __launch_bounds__(256, (StripWaves<LA, LB, LC, LD>::value)) void jk_s
   double* sa = strip;
   double* sb = strip + NA * n;
   for (int k = tid; k < (NA + NB) * n; k += blockDim.x) strip[k] = 0.0;
+  const bool ldsb = strip_lds_boys<LA, LB, LC, LD>(a.lds_n);
+  double* btab = strip + (NA + NB) * a.lds_n;
+  if constexpr (StripLdsBoys<LA, LB, LC, LD>::value) {
+    if (ldsb) {
+      constexpr int L = LA + LB + LC + LD;
+      for (int k = tid; k < kBoysRows * kLdsBoysCols; k += blockDim.x) {
+        const int r = k / kLdsBoysCols, c = k % kLdsBoysCols;
+        btab[k] = a.boys[r * kBoysCols + (c == 0 ? 0 : L + c)];
+      }
+    }
+  }
   {
     const gcptr<double> src = prim + (size_t)P.poff * kPrimPairStride;
     for (int k = tid; k < P.nprim * kPrimPairStride; k += blockDim.x) bpd[k] = src[k];
@@ -595,9 +630,20 @@ __global__ __launch_bounds__(256, (StripWaves<LA, LB, LC, LD>::value)) void jk_s
       }
     }
     if (!lo) {
-      contracted_quartet<LA, LB, LC, LD>((const double*)bpd, P.nprim,
-                                         prim + (size_t)K.poff * kPrimPairStride, K.nprim, a.boys,
-                                         P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
+      bool done = false;
+      if constexpr (StripLdsBoys<LA, LB, LC, LD>::value) {
+        if (ldsb) {
+          contracted_quartet_body<LA, LB, LC, LD, kLdsBoysCols, 1>((const double*)bpd, P.nprim,
+                                                                   prim + (size_t)K.poff * kPrimPairStride,
+                                                                   K.nprim, (const double*)btab, P.ABx, P.ABy,
+                                                                   P.ABz, K.ABx, K.ABy, K.ABz, out);
+          done = true;
+        }
+      }
+      if (!done)
+        contracted_quartet<LA, LB, LC, LD>((const double*)bpd, P.nprim,
+                                           prim + (size_t)K.poff * kPrimPairStride, K.nprim, a.boys,
+                                           P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
     }
     const double deg = sbra * (K.sa != K.sb ? 2.0 : 1.0) * ((a.same_class && j == t.bra) ? 1.0 : 2.0);
     double nc[NC], nd[ND];
@@ -848,7 +894,8 @@ static void launch_jk_t(const JKArgs& a, hipStream_t s) {
 template <int LA, int LB, int LC, int LD>
 static void launch_jk_strip_t(const JKArgs& a, hipStream_t s) {
   if (a.ntasks <= 0) return;
-  const size_t lds = (size_t)(ncart(LA) + ncart(LB)) * a.lds_n * sizeof(double);
+  const size_t lds = (size_t)(ncart(LA) + ncart(LB)) * a.lds_n * sizeof(double) +
+                     (strip_lds_boys<LA, LB, LC, LD>(a.lds_n) ? kLdsBoysBytes : 0);
   hipLaunchKernelGGL((jk_strip_kernel<LA, LB, LC, LD>), dim3(a.ntasks), dim3(a.block > 0 ? a.block : 256), lds, s,
                      a);
 }
