@@ -525,11 +525,14 @@ template <> struct StripLdsBoys<1, 1, 0, 0> { static constexpr bool value = true
 constexpr int kLdsBoysCols = 9;
 constexpr size_t kLdsBoysBytes = (size_t)kBoysRows * kLdsBoysCols * sizeof(double);
 
-// Whether a strip launch of this class with row strips of width lds_n keeps
-// the Boys columns in LDS (host and device agree through this function).
+// Whether a strip launch of this class with row strips of width lds_n and
+// workgroups of `block` threads keeps the Boys columns in LDS (host and
+// device agree through this function).  Only 256-thread workgroups, which
+// run long ket streams: a one-wavefront workgroup of a small molecule
+// (a few dozen kets) would spend more on copying the table than it saves.
 template <int LA, int LB, int LC, int LD>
-__host__ __device__ constexpr bool strip_lds_boys(int lds_n) {
-  return StripLdsBoys<LA, LB, LC, LD>::value &&
+__host__ __device__ constexpr bool strip_lds_boys(int lds_n, int block) {
+  return StripLdsBoys<LA, LB, LC, LD>::value && (block <= 0 || block >= 256) &&
          (size_t)(ncart(LA) + ncart(LB)) * (size_t)lds_n * sizeof(double) + kLdsBoysBytes <= 65536;
 }
 
@@ -566,7 +569,7 @@ __global__ __launch_bounds__(256, (StripWaves<LA, LB, LC, LD>::value)) void jk_s
   double* sa = strip;
   double* sb = strip + NA * n;
   for (int k = tid; k < (NA + NB) * n; k += blockDim.x) strip[k] = 0.0;
-  const bool ldsb = strip_lds_boys<LA, LB, LC, LD>(a.lds_n);
+  const bool ldsb = strip_lds_boys<LA, LB, LC, LD>(a.lds_n, a.block);
   double* btab = strip + (NA + NB) * a.lds_n;
   if constexpr (StripLdsBoys<LA, LB, LC, LD>::value) {
     if (ldsb) {
@@ -895,7 +898,7 @@ template <int LA, int LB, int LC, int LD>
 static void launch_jk_strip_t(const JKArgs& a, hipStream_t s) {
   if (a.ntasks <= 0) return;
   const size_t lds = (size_t)(ncart(LA) + ncart(LB)) * a.lds_n * sizeof(double) +
-                     (strip_lds_boys<LA, LB, LC, LD>(a.lds_n) ? kLdsBoysBytes : 0);
+                     (strip_lds_boys<LA, LB, LC, LD>(a.lds_n, a.block) ? kLdsBoysBytes : 0);
   hipLaunchKernelGGL((jk_strip_kernel<LA, LB, LC, LD>), dim3(a.ntasks), dim3(a.block > 0 ? a.block : 256), lds, s,
                      a);
 }
