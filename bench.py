@@ -347,10 +347,18 @@ def small_configs_bench(H, LI, S, WF, steps, stream):
     G1 = dG.download().reshape(n, n, order="F")
     F1 = hcore[0] + G1
     Gh = h.compute_2body_intraspecies_direct(D0, -0.5)
+    from openlowdin_amd import scf_device as SD
+    t0 = time.perf_counter()
+    scf = SD.DeviceSCF(S.water("6-31G**"))
+    E0 = scf.run(e_tol=1e-10, d_tol=1e-8)
+    t_scf = time.perf_counter() - t0
     out["C0_water"] = {"n": n, "builds_per_s": round(1.0 / s0, 1), "ms_per_build": round(s0 * 1e3, 4),
                        "eris_per_build": h.stats()["eris"],
                        "first_iteration_fock": {"trace": float(np.trace(F1)), "max_abs": float(np.max(np.abs(F1))),
-                                                "max_abs_diff_vs_host_entry": float(np.max(np.abs(G1 - Gh)))}}
+                                                "max_abs_diff_vs_host_entry": float(np.max(np.abs(G1 - Gh)))},
+                       "scf": {"energy": E0, "iterations": scf.iterations, "converged": bool(scf.converged),
+                               "seconds": round(t_scf, 3), "note": "device-resident SCF, H-core guess, DIIS, "
+                                                                   "|dE| < 1e-10 and density change < 1e-8"}}
     f = S.fhf_anion("6-311++G**")
     e, pr = f.species
     he, hp = LI.Libint2Interface(e, 1), LI.Libint2Interface(pr, 2)
