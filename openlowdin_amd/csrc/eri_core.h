@@ -191,19 +191,22 @@ __device__ __forceinline__ void contracted_quartet_body(BP bprim, int nbp, KP kp
 // FP32 Boys: fifth-order Taylor on the full table's grid (|dT| <= 0.05,
 // error ~3e-9 relative), exp(-T) the same way, downward recursion; above
 // T = 40 the asymptotic form with upward recursion.
-template <int L>
-__device__ __forceinline__ void boys_f32(float T, const float* __restrict__ tab, float* F) {
-  static_assert(1 + L + kBoysF32Taylor - 1 < kBoysF32Cols, "FP32 Boys table lacks the Taylor columns of this L");
+// RS floats per row, F_L .. F_{L+4} at columns FO .. FO+4 (the full table:
+// RS = kBoysF32Cols, FO = 1 + L; a per-class copy in LDS: RS = 6, FO = 1).
+template <int L, int RS = kBoysF32Cols, int FO = 1 + L, typename TP>
+__device__ __forceinline__ void boys_f32(float T, TP tab, float* F) {
+  static_assert(RS != kBoysF32Cols || 1 + L + kBoysF32Taylor - 1 < kBoysF32Cols,
+                "FP32 Boys table lacks the Taylor columns of this L");
   if (T < kBoysF32Tmax) {
     // T < Tmax rounds to a row <= Tmax * InvH, the table's last row
     const int i = (int)(T * kBoysF32InvH + 0.5f);
     const float x = (float)i * kBoysF32H - T;
-    const float* r = tab + i * kBoysF32Cols;
-    float s = r[1 + L + 4];
-    s = fmaf(s, x * 0.25f, r[1 + L + 3]);
-    s = fmaf(s, x * (1.0f / 3.0f), r[1 + L + 2]);
-    s = fmaf(s, x * 0.5f, r[1 + L + 1]);
-    s = fmaf(s, x, r[1 + L + 0]);
+    const auto r = tab + i * RS;
+    float s = r[FO + 4];
+    s = fmaf(s, x * 0.25f, r[FO + 3]);
+    s = fmaf(s, x * (1.0f / 3.0f), r[FO + 2]);
+    s = fmaf(s, x * 0.5f, r[FO + 1]);
+    s = fmaf(s, x, r[FO + 0]);
     F[L] = s;
     if (L > 0) {
       float e = 1.0f;
@@ -230,11 +233,11 @@ __device__ __forceinline__ void boys_f32(float T, const float* __restrict__ tab,
 // FP32 contracted quartet (mixed precision): same loop structure as the
 // FP64 one over FP32 copies of the primitive-pair records; the integrals
 // come back as FP64 for the FP64 contraction and accumulation.
-template <int LA, int LB, int LC, int LD, typename BP, typename KP>
-__device__ __forceinline__ void contracted_quartet_f32_body(BP bprim, int nbp, KP kprim, int nkp,
-                                                       const float* __restrict__ boys, double ABx, double ABy,
-                                                       double ABz, double CDx, double CDy, double CDz,
-                                                       double* out) {
+template <int LA, int LB, int LC, int LD, int RS = kBoysF32Cols, int FO = 1 + LA + LB + LC + LD, typename BP,
+          typename KP, typename TP>
+__device__ __forceinline__ void contracted_quartet_f32_body(BP bprim, int nbp, KP kprim, int nkp, TP boys,
+                                                       double ABx, double ABy, double ABz, double CDx,
+                                                       double CDy, double CDz, double* out) {
   using C = EriClass<LA, LB, LC, LD>;
   constexpr int L = LA + LB + LC + LD;
   constexpr int NOUT = ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD);
@@ -258,7 +261,7 @@ __device__ __forceinline__ void contracted_quartet_f32_body(BP bprim, int nbp, K
       const float T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
       const float pref = bp[8] * Kcd * rs;
       float F[L + 1];
-      boys_f32<L>(T, boys, F);
+      boys_f32<L, RS, FO>(T, boys, F);
 #pragma unroll
       for (int m = 0; m <= L; ++m) F[m] *= pref;
       C::template vrr<float>(F, bp[5], bp[6], bp[7], -rop * PQx, -rop * PQy, -rop * PQz, QCx, QCy, QCz,
@@ -524,6 +527,9 @@ template <> struct StripLdsBoys<1, 0, 0, 0> { static constexpr bool value = true
 template <> struct StripLdsBoys<1, 1, 0, 0> { static constexpr bool value = true; };
 constexpr int kLdsBoysCols = 9;
 constexpr size_t kLdsBoysBytes = (size_t)kBoysRows * kLdsBoysCols * sizeof(double);
+// FP32 copy for the mixed-precision quartets: e^-T and F_L .. F_{L+4}.
+constexpr int kLdsBoys32Cols = 6;
+constexpr size_t kLdsBoys32Bytes = (size_t)kBoysF32Rows * kLdsBoys32Cols * sizeof(float);
 
 // Whether a strip launch of this class with row strips of width lds_n and
 // workgroups of `block` threads keeps the Boys columns in LDS (host and
@@ -534,6 +540,14 @@ template <int LA, int LB, int LC, int LD>
 __host__ __device__ constexpr bool strip_lds_boys(int lds_n, int block) {
   return StripLdsBoys<LA, LB, LC, LD>::value && (block <= 0 || block >= 256) &&
          (size_t)(ncart(LA) + ncart(LB)) * (size_t)lds_n * sizeof(double) + kLdsBoysBytes <= 65536;
+}
+// The same for the FP32 columns, when some molecule of the launch runs mixed
+// precision and both copies fit.
+template <int LA, int LB, int LC, int LD>
+__host__ __device__ constexpr bool strip_lds_boys32(int lds_n, int block, int any_mixed) {
+  return strip_lds_boys<LA, LB, LC, LD>(lds_n, block) && any_mixed != 0 &&
+         (size_t)(ncart(LA) + ncart(LB)) * (size_t)lds_n * sizeof(double) + kLdsBoysBytes + kLdsBoys32Bytes <=
+             65536;
 }
 
 // ------------------------------------------------ J/K kernel, LDS strips
@@ -571,12 +585,20 @@ __global__ __launch_bounds__(256, (StripWaves<LA, LB, LC, LD>::value)) void jk_s
   for (int k = tid; k < (NA + NB) * n; k += blockDim.x) strip[k] = 0.0;
   const bool ldsb = strip_lds_boys<LA, LB, LC, LD>(a.lds_n, a.block);
   double* btab = strip + (NA + NB) * a.lds_n;
+  const bool ldsb32 = kF32 && strip_lds_boys32<LA, LB, LC, LD>(a.lds_n, a.block, a.any_mixed);
+  float* btab32 = reinterpret_cast<float*>(btab + kBoysRows * kLdsBoysCols);
   if constexpr (StripLdsBoys<LA, LB, LC, LD>::value) {
+    constexpr int L = LA + LB + LC + LD;
     if (ldsb) {
-      constexpr int L = LA + LB + LC + LD;
       for (int k = tid; k < kBoysRows * kLdsBoysCols; k += blockDim.x) {
         const int r = k / kLdsBoysCols, c = k % kLdsBoysCols;
         btab[k] = a.boys[r * kBoysCols + (c == 0 ? 0 : L + c)];
+      }
+    }
+    if (ldsb32) {
+      for (int k = tid; k < kBoysF32Rows * kLdsBoys32Cols; k += blockDim.x) {
+        const int r = k / kLdsBoys32Cols, c = k % kLdsBoys32Cols;
+        btab32[k] = a.boysf[r * kBoysF32Cols + (c == 0 ? 0 : L + c)];
       }
     }
   }
@@ -626,10 +648,19 @@ __global__ __launch_bounds__(256, (StripWaves<LA, LB, LC, LD>::value)) void jk_s
     if constexpr (kF32) {
       if (lo) {
         myq32 += 1;
-        contracted_quartet_f32<LA, LB, LC, LD>((const float*)bpf, P.nprim,
-                                               (gcptr<float>)M.primf + (size_t)K.poff * kPrimPairStride, K.nprim,
-                                               a.boysf,
-                                               P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
+        bool done = false;
+        if constexpr (StripLdsBoys<LA, LB, LC, LD>::value) {
+          if (ldsb32) {
+            contracted_quartet_f32_body<LA, LB, LC, LD, kLdsBoys32Cols, 1>(
+                (const float*)bpf, P.nprim, (gcptr<float>)M.primf + (size_t)K.poff * kPrimPairStride, K.nprim,
+                (const float*)btab32, P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
+            done = true;
+          }
+        }
+        if (!done)
+          contracted_quartet_f32<LA, LB, LC, LD>((const float*)bpf, P.nprim,
+                                                 (gcptr<float>)M.primf + (size_t)K.poff * kPrimPairStride,
+                                                 K.nprim, a.boysf, P.ABx, P.ABy, P.ABz, K.ABx, K.ABy, K.ABz, out);
       }
     }
     if (!lo) {
@@ -898,7 +929,8 @@ template <int LA, int LB, int LC, int LD>
 static void launch_jk_strip_t(const JKArgs& a, hipStream_t s) {
   if (a.ntasks <= 0) return;
   const size_t lds = (size_t)(ncart(LA) + ncart(LB)) * a.lds_n * sizeof(double) +
-                     (strip_lds_boys<LA, LB, LC, LD>(a.lds_n, a.block) ? kLdsBoysBytes : 0);
+                     (strip_lds_boys<LA, LB, LC, LD>(a.lds_n, a.block) ? kLdsBoysBytes : 0) +
+                     (strip_lds_boys32<LA, LB, LC, LD>(a.lds_n, a.block, a.any_mixed) ? kLdsBoys32Bytes : 0);
   hipLaunchKernelGGL((jk_strip_kernel<LA, LB, LC, LD>), dim3(a.ntasks), dim3(a.block > 0 ? a.block : 256), lds, s,
                      a);
 }

@@ -535,12 +535,12 @@ class CoopGen:
             hlevel[key] = 0 if op[0] == "acc" else 1 + max(hlev(op[2]), hlev(op[3]))
             return hlevel[key]
 
-        real = set()
+        real = {}     # insertion-ordered: the generated tables do not depend on string hashing
         for k in list(hops):
             rk = resolve(k)
             hlev(rk)
             if hops[rk][0] != "acc":
-                real.add(rk)
+                real[rk] = None
         final = {}
         for k, idx in outs.items():
             final[resolve(k)] = idx

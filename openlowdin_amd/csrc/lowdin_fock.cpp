@@ -254,6 +254,7 @@ struct BuildPlan {
   std::vector<size_t> off, shoff;       // packed n^2 / nshell^2 offsets; last = total
   int nmax = 0, nshmax = 0;
   bool any_dn = false;                  // some molecule screens or runs mixed
+  bool any_mixed = false;               // some molecule runs mixed precision
   std::vector<MolDesc> desc, desc_dev;  // wanted / on the device
   std::vector<SymDesc> sym, sym_dev;
   DevBuf<MolDesc> d_desc;
@@ -281,6 +282,7 @@ struct BuildPlan {
   uint64_t graph_layout = ~0ull;
   hipStream_t graph_stream = nullptr;
   bool graph_any_dn = false;
+  bool graph_any_mixed = false;
   double graph_eps = 0.0;
 
   BuildPlan() {
@@ -807,6 +809,7 @@ static void enqueue_classes(BuildPlan& p, hipStream_t st, bool timed, int which,
     a.block = L.strip == 2 ? 64 : 256;
     if (L.strip == 4) a.lds_n = 0;
     a.eps = p.eps;  // DBL_EPSILON by default (Libint2Iface.h:269-271)
+    a.any_mixed = p.any_mixed ? 1 : 0;
     if (timed) HIPCHK(hipEventRecord(p.launch_ev[2 * k], ls));
     const bool ok = L.strip >= 3 ? dispatch_jk_coop(L.la, L.lb, L.lc, L.ld, a, ls)
                     : L.strip    ? dispatch_jk_strip(L.la, L.lb, L.lc, L.ld, a, ls)
@@ -845,6 +848,7 @@ static void enqueue_plan(BuildPlan& p, const double* dD, double* dG, const doubl
   p.desc.resize(nm);
   p.sym.resize(nm);
   p.any_dn = false;
+  p.any_mixed = false;
   p.eps = nm > 0 ? p.mols[0]->screen_eps : p.eps;
   for (int m = 1; m < nm; ++m) p.eps = std::min(p.eps, p.mols[m]->screen_eps);
   for (int m = 0; m < nm; ++m) {
@@ -867,6 +871,7 @@ static void enqueue_plan(BuildPlan& p, const double* dD, double* dG, const doubl
     M.half_factor = 0.5 * factors[m];
     M.tau = s.tau;
     p.any_dn = p.any_dn || M.screen || M.mixed;
+    p.any_mixed = p.any_mixed || M.mixed;
     SymDesc& Y = p.sym[m];
     std::memset(&Y, 0, sizeof(Y));
     Y.Graw = M.Graw;
@@ -891,6 +896,7 @@ static void enqueue_plan(BuildPlan& p, const double* dD, double* dG, const doubl
     // enqueued after the prologue and before the graph launch, and joined
     // with it before the epilogue, so that they run beside the graph.
     if (!p.graph || p.graph_layout != p.layout || p.graph_stream != st || p.graph_any_dn != p.any_dn ||
+        p.graph_any_mixed != p.any_mixed ||
         p.graph_eps != p.eps) {
       if (p.graph) HIPCHK(hipGraphExecDestroy(p.graph));
       p.graph = nullptr;
@@ -910,6 +916,7 @@ static void enqueue_plan(BuildPlan& p, const double* dD, double* dG, const doubl
       p.graph_layout = p.layout;
       p.graph_stream = st;
       p.graph_any_dn = p.any_dn;
+      p.graph_any_mixed = p.any_mixed;
       p.graph_eps = p.eps;
     }
     enqueue_prologue(p, st);

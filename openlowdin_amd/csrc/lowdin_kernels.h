@@ -41,6 +41,7 @@ struct JKArgs {
   int lds_n;               // largest n of the launch (LDS strip sizing)
   int block;               // strip kernel: threads per workgroup (0: 256)
   double eps;
+  int any_mixed;           // some molecule of the launch runs mixed precision
 };
 
 // Per molecule of a batched symmetrisation.
