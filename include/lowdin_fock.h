@@ -346,7 +346,7 @@ int lowdin_fock_targets(void* lint, int* targets);
  * iteration whose matrices stay in HBM (openLOWDIN src/scf/MultiSCF.f90:
  * 276-440, 730-1002; WaveFunction.f90:394-467, 817-1535, 1768-2003).  Per
  * species: H-core guess, canonical orthogonaliser (overlap eigenvalues
- * > 1e-8), G = q^2 (J + f K) through this library's direct build (skipped
+ * >= OVERLAP_EIGEN_THRESHOLD, 1e-8 unless set), G = q^2 (J + f K) through this library's direct build (skipped
  * when build_two = 0), couplings q_i q_j J_{i<-j} through the device coupling
  * build, F = H + G + C, DIIS (8) on X^T (F D S - S D F) X over all species,
  * rocSOLVER diagonalisation, D = eta C_occ C_occ^T.  E = e_pc +
@@ -368,6 +368,10 @@ void* lowdin_scf_new(int nspecies, void* const* lints, const lowdin_scf_species*
 void lowdin_scf_del(void* scf);
 /* Core Hamiltonian and overlap of a species (host, n x n column-major). */
 int lowdin_scf_set_onebody(void* scf, int species, const double* H, const double* S);
+/* Overlap eigenvalues kept by the canonical orthogonaliser from the next
+ * lowdin_scf_start on (CONTROL OVERLAP_EIGEN_THRESHOLD, default 1e-8,
+ * CONTROL.f90:1513, used at WaveFunction.f90:419).  threshold > 0. */
+int lowdin_scf_set_overlap_threshold(void* scf, double threshold);
 /* Orthogonalisers and H-core guess densities; clears the DIIS history. */
 int lowdin_scf_start(void* scf);
 /* Fock matrices, energy and DIIS error (max |X^T (F D S - S D F) X|) from the

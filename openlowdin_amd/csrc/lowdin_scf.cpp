@@ -4,7 +4,8 @@
 // Follows openLOWDIN's SCF (src/scf/MultiSCF.f90:276-440, 730-1002;
 // SingleSCF.f90:146-252; WaveFunction.f90:394-467, 817-1535, 1768-2003):
 //   * orthogonaliser X = U s^{-1/2} over the overlap eigenpairs with
-//     s > 1e-8 (canonical orthogonalisation; WaveFunction.f90:394-467);
+//     s >= OVERLAP_EIGEN_THRESHOLD (1e-8 unless set; canonical
+//     orthogonalisation, WaveFunction.f90:394-467, 419);
 //   * H-core guess: D from the eigenvectors of X^T H X;
 //   * per iteration and species: G = J[D] + f K[D] (the direct build,
 //     skipped for one particle unless requested), times q^2; the couplings
@@ -95,6 +96,7 @@ constexpr int kScal = 5 + kMaxDiis;   // ..., [4] info of the last diagonalisati
 struct Scf {
   std::vector<ScfSpecies> sp;
   double e_pc = 0.0;
+  double overlap_threshold = 1e-8;   // OVERLAP_EIGEN_THRESHOLD (CONTROL.f90:1513)
   hipStream_t stream = nullptr;
   rocblas_handle blas = nullptr;
   int* d_info = nullptr;
@@ -184,7 +186,7 @@ void start(Scf& s) {
     SCFCHK(hipMemcpyAsync(w.data(), p.W.p, n * sizeof(double), hipMemcpyDeviceToHost, s.stream));
     SCFCHK(hipStreamSynchronize(s.stream));
     int col0 = 0;
-    while (col0 < n && !(w[col0] > 1e-8)) ++col0;
+    while (col0 < n && !(w[col0] >= s.overlap_threshold)) ++col0;
     p.m = n - col0;
     if (p.m < p.nocc) throw std::runtime_error("fewer orthogonal functions than occupied orbitals");
     lowdin::launch_scale_cols(p.X.p, p.T.p, p.W.p, n, p.m, col0, s.stream);
@@ -504,6 +506,18 @@ int lowdin_scf_set_onebody(void* scf, int species, const double* H, const double
     return 0;
   } catch (const std::exception& e) {
     lowdin::set_last_error(std::string("scf_set_onebody: ") + e.what());
+    return -1;
+  }
+}
+
+int lowdin_scf_set_overlap_threshold(void* scf, double threshold) {
+  try {
+    if (!(threshold > 0.0)) throw std::runtime_error("threshold must be positive");
+    as_scf(scf)->overlap_threshold = threshold;
+    lowdin::clear_last_error();
+    return 0;
+  } catch (const std::exception& e) {
+    lowdin::set_last_error(std::string("scf_set_overlap_threshold: ") + e.what());
     return -1;
   }
 }

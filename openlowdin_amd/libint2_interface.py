@@ -134,6 +134,8 @@ def lib():
         L.lowdin_scf_del.argtypes = [vp]
         L.lowdin_scf_set_onebody.argtypes = [vp, C.c_int, dp, dp]
         L.lowdin_scf_set_onebody.restype = C.c_int
+        L.lowdin_scf_set_overlap_threshold.argtypes = [vp, C.c_double]
+        L.lowdin_scf_set_overlap_threshold.restype = C.c_int
         L.lowdin_scf_start.argtypes = [vp]
         L.lowdin_scf_start.restype = C.c_int
         L.lowdin_scf_fock.argtypes = [vp, C.c_int, dp, dp]

@@ -20,7 +20,7 @@ from . import wavefunction as WF
 class DeviceSCF:
     def __init__(self, system, objs: Optional[Dict[int, LI.Libint2Interface]] = None,
                  build_for_one_particle: bool = False, stream: int = 0, hcore=None, overlap=None,
-                 ghost: str = ""):
+                 ghost: str = "", overlap_threshold: float = 1e-8):
         self.system = system
         self.objs = {} if objs is None else objs
         ns = len(system.species)
@@ -40,6 +40,8 @@ class DeviceSCF:
         if not self.this:
             LI._check("scf_new")
             raise LI.LowdinFockError("scf_new failed")
+        if LI.lib().lowdin_scf_set_overlap_threshold(self.this, float(overlap_threshold)) != 0:
+            LI._check("scf_set_overlap_threshold")
         if hcore is None or overlap is None:
             hcore, overlap = WF.one_body_matrices(system, self.objs)
         self.n = [h.nbasis for h in self.handles]

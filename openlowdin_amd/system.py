@@ -142,16 +142,23 @@ PROTON_MASS = 1836.15267247
 def build_system(atoms: Sequence[Tuple[str, Sequence[float]]], e_basis: str,
                  quantum: Sequence[Tuple[str, str, Sequence[float], float]] = (),
                  extra_electrons: int = 0, open_shell: bool = False,
-                 angstrom: bool = True) -> MolecularSystem:
+                 angstrom: bool = True, add_particles: Optional[Dict[str, int]] = None) -> MolecularSystem:
     """Build a system: electrons in ``e_basis`` on every atom of ``atoms``
     (element, xyz); point charges for every atom not listed as quantum;
     ``quantum`` entries are (species symbol, basis, xyz, mass) quantum nuclei,
-    one particle each; entries with the same symbol form one species.
+    one particle each, or positron basis centres (symbol E+, one particle
+    each; a nucleus keeps its point charge beside them); entries with the
+    same symbol form one species.  ``add_particles`` adds to a quantum
+    species' particle count, the addParticles keyword of a GEOMETRY line
+    (Particle.f90:157; three E+ centres with addParticles=-2 hold one
+    positron).
 
     With ``open_shell`` the electrons are split into E-ALPHA / E-BETA species
     (eta = 1), as openLOWDIN's UHF does."""
     f = 1.0 / ANGSTROM if angstrom else 1.0
-    qpos = [np.asarray(x, float) * f for (_, _, x, _) in quantum]
+    # a quantum nucleus replaces the point charge at its position; positrons
+    # (E+) sit beside the nuclei
+    qpos = [np.asarray(x, float) * f for (sym, _, x, _) in quantum if not sym.upper().startswith("E+")]
     e_shells: List[Shell] = []
     charges = []
     nelec = extra_electrons
@@ -182,6 +189,10 @@ def build_system(atoms: Sequence[Tuple[str, Sequence[float]]], e_basis: str,
         sp = by_symbol[sym]
         sp.n_particles += 1
         sp.shells += basis_shells(bas, sym, xb)
+    for sym, extra in (add_particles or {}).items():
+        if sym not in by_symbol or by_symbol[sym].n_particles + extra < 1:
+            raise ValueError("add_particles: no quantum species %r or fewer than one particle" % sym)
+        by_symbol[sym].n_particles += extra
     return MolecularSystem(species, charges)
 
 

@@ -86,7 +86,7 @@ def one_body_matrices(system, objs):
 
 def _sym_orth(S, thresh=1e-8):
     w, U = np.linalg.eigh(S)
-    keep = w > thresh
+    keep = w >= thresh
     return U[:, keep] / np.sqrt(w[keep])
 
 
@@ -110,7 +110,8 @@ def damping_step(Finit, Dinit, Fnew, Dnew, zero: float = 1e-12):
 def run_scf(system, hcore: Optional[List[np.ndarray]] = None, overlap: Optional[List[np.ndarray]] = None,
             max_iter: int = 300,
             e_tol: float = 1e-11, d_tol: float = 1e-9, objs=None, build_for_one_particle: bool = False,
-            verbose: bool = False, guess=None, history=None, convergence: str = "diis", ghost: str = ""):
+            verbose: bool = False, guess=None, history=None, convergence: str = "diis", ghost: str = "",
+            overlap_threshold: float = 1e-8):
     """Multi-species HF with GPU Fock builds.  hcore/overlap per species
     (normalised Cartesian basis; computed on the GPU when not given).
     guess: starting densities (None entries: H-core guess); history: list
@@ -122,12 +123,14 @@ def run_scf(system, hcore: Optional[List[np.ndarray]] = None, overlap: Optional[
     ENERGY criterion, MultiSCF.f90:795-830).  ghost: name of a species whose
     coupling is left out of the other species' Fock matrices but kept in the
     energy (SCF_GHOST_SPECIES, WaveFunction.f90:1502-1503, 1980-1986).
-    Returns (energy, wavefunctions)."""
+    overlap_threshold: overlap eigenvalues kept by the canonical
+    orthogonaliser (OVERLAP_EIGEN_THRESHOLD, default 1e-8, WaveFunction.f90:
+    419).  Returns (energy, wavefunctions)."""
     objs = {} if objs is None else objs
     if hcore is None or overlap is None:
         hcore, overlap = one_body_matrices(system, objs)
     wfs = [WaveFunction(system, k, objs) for k in range(len(system.species))]
-    X = [_sym_orth(S) for S in overlap]
+    X = [_sym_orth(S, overlap_threshold) for S in overlap]
 
     def density(k, F):
         w, Cp = np.linalg.eigh(X[k].T @ F @ X[k])

@@ -23,12 +23,12 @@ import oracle as O
 
 def _sym_orth(S: np.ndarray, thresh: float = 1e-8) -> np.ndarray:
     w, U = np.linalg.eigh(S)
-    keep = w > thresh
+    keep = w >= thresh
     return U[:, keep] @ np.diag(w[keep] ** -0.5) @ U[:, keep].T if keep.all() else U[:, keep] / np.sqrt(w[keep])
 
 
 class SpeciesState:
-    def __init__(self, sid, species, charges):
+    def __init__(self, sid, species, charges, overlap_threshold: float = 1e-8):
         self.sid = sid
         self.sp = species
         self.basis = O.Basis(species)
@@ -37,7 +37,7 @@ class SpeciesState:
         V = O.one_body(self.basis, 3, charges)  # sum_C -Z_C/|r-C|
         self.S = S
         self.H = T / species.mass - species.charge * V
-        self.X = _sym_orth(S)
+        self.X = _sym_orth(S, overlap_threshold)
         self.pairs, _ = O.shellpair_list(self.basis)
         self.Q = O.schwarz(self.basis)
         self.D = None
@@ -98,15 +98,17 @@ def damping_step(Finit, Dinit, Fnew, Dnew, zero=1e-12):
 
 def run_scf(system, max_iter: int = 300, e_tol: float = 1e-11, d_tol: float = 1e-9,
             build_for_one: bool = False, verbose: bool = False, guess=None, history=None,
-            convergence: str = "diis"):
+            convergence: str = "diis", overlap_threshold: float = 1e-8):
     """guess: optional list of starting densities (None entries: H-core);
     history: optional list that receives the energy of every iteration.
     convergence: "diis" (DIIS(8) on X^T (F D S - S D F) X, converged when
     |dE| < e_tol and the error < d_tol) or "damping" (the reference's
     default: CONVERGENCE_METHOD=1, ITERATION_SCHEME=3, one damped update of
     every species per global iteration, converged when |dE| < e_tol, the
-    ENERGY criterion; MultiSCF.f90:276-440, 795-830)."""
-    states = [SpeciesState(k + 1, sp, system.charges) for k, sp in enumerate(system.species)]
+    ENERGY criterion; MultiSCF.f90:276-440, 795-830).  overlap_threshold:
+    overlap eigenvalues kept by the canonical orthogonaliser
+    (OVERLAP_EIGEN_THRESHOLD, default 1e-8, WaveFunction.f90:419)."""
+    states = [SpeciesState(k + 1, sp, system.charges, overlap_threshold) for k, sp in enumerate(system.species)]
     for k, st in enumerate(states):
         if guess is not None and guess[k] is not None:
             st.D = np.array(guess[k], dtype=np.float64)

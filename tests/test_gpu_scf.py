@@ -5,7 +5,8 @@ two-particle and coupling build on the GPU through the C ABI (host linear
 algebra for the orbital update).  The converged total energies must match
 the energies the reference's test checkers hold, to 1e-9 Eh (the checkers
 themselves allow 1e-8): test/H2O.BOA.DIRECT.py:18, test/FHF-.NOCI-RHF.py:19,
-test/H2O.APMO.UHF.DIRECT.py:19-20, test/HCN.py:19."""
+test/H2O.APMO.UHF.DIRECT.py:19-20, test/HCN.py:19, test/HCN.e+.py:19,
+test/HCN.e+.FT.py:19."""
 import numpy as np
 import pytest
 
@@ -86,6 +87,33 @@ def test_reference_energy_device_scf(name, builder, eref):
     print("   E = %.12f  ref %.12f  diff %.2e  (%d iterations)" % (E, eref, E - eref, scf.iterations))
     assert scf.converged
     assert abs(E - eref) <= 1e-9
+
+
+HCN_ATOMS = [("H", (0, 0, 0)), ("C", (0, 0, 1.059)), ("N", (0, 0, 2.186))]
+
+
+@pytest.mark.parametrize("driver", ["host", "device"])
+@pytest.mark.parametrize("ghost,eref,tol", [("", -92.901811855736, 1e-9), ("E+", -92.901807583131, 1e-8)],
+                         ids=["HCN.e+", "HCN.e+.FT"])
+def test_reference_energy_positron_on_gpu(driver, ghost, eref, tol):
+    """HCN with one positron (GRIBAKIN-10S on the three nuclei, addParticles
+    =-2) and overlapEigenThreshold=1E-6 (at 1e-8 the energy moves by 3e-6):
+    test/HCN.e+.lowdin, and test/HCN.e+.FT.lowdin with the positron as
+    SCF ghost species (its coupling left out of the electrons' Fock matrix).
+    The reference's checker allows 1e-8 for HCN.e+.FT."""
+    from openlowdin_amd import scf_device as SD
+    from openlowdin_amd import wavefunction as WF
+    sysm = S.build_system(HCN_ATOMS, "6-311++G.D.P", quantum=[("E+", "GRIBAKIN-10S", x, 1.0) for _, x in HCN_ATOMS],
+                          add_particles={"E+": -2})
+    _case("%s SCF: HCN.e+ ghost=%r" % (driver, ghost))
+    if driver == "host":
+        E, _ = WF.run_scf(sysm, e_tol=1e-11, d_tol=1e-8, ghost=ghost, overlap_threshold=1e-6)
+    else:
+        scf = SD.DeviceSCF(sysm, ghost=ghost, overlap_threshold=1e-6)
+        E = scf.run(e_tol=1e-11, d_tol=1e-8)
+        assert scf.converged
+    print("   E = %.12f  ref %.12f  diff %.2e" % (E, eref, E - eref))
+    assert abs(E - eref) <= tol
 
 
 def test_read_coefficients_energy_on_gpu():

@@ -224,6 +224,32 @@ def test_reference_energy_hcn():
     assert abs(E - (-92.901743215468)) <= 1e-8
 
 
+HCN_ATOMS = [("H", (0, 0, 0)), ("C", (0, 0, 1.059)), ("N", (0, 0, 2.186))]
+HCN_POSITRON = [("E+", "GRIBAKIN-10S", x, 1.0) for _, x in HCN_ATOMS]
+
+
+def test_hcn_positron_system():
+    """test/HCN.e+.lowdin: positron centres on the nuclei keep the nuclei's
+    point charges, and addParticles=-2 on three centres leaves one positron."""
+    sysm = S.build_system(HCN_ATOMS, "6-311++G.D.P", quantum=HCN_POSITRON, add_particles={"E+": -2})
+    assert [q for q, _ in sysm.charges] == [1.0, 6.0, 7.0]
+    assert [sp.n_particles for sp in sysm.species] == [14, 1]
+    assert sysm.species[1].charge == 1.0 and sysm.species[1].nbasis == 30
+    with pytest.raises(ValueError):
+        S.build_system(HCN_ATOMS, "6-311++G.D.P", quantum=HCN_POSITRON, add_particles={"E+": -3})
+
+
+def test_reference_energy_hcn_positron():
+    """test/HCN.e+.py:19: RHF electrons (6-311++G(d,p)) and one positron
+    (GRIBAKIN-10S on the three nuclei), overlapEigenThreshold=1E-6:
+    -92.901811855736 (the reference's run stops at totalEnergyTolerance
+    1E-8 with damping; DIIS converged further lands within 1e-9)."""
+    import scf
+    sysm = S.build_system(HCN_ATOMS, "6-311++G.D.P", quantum=HCN_POSITRON, add_particles={"E+": -2})
+    E, _ = scf.run_scf(sysm, e_tol=1e-11, d_tol=1e-8, overlap_threshold=1e-6)
+    assert abs(E - (-92.901811855736)) <= 1e-9
+
+
 @pytest.mark.slow
 def test_reference_damping_h2o_apmo_iter3():
     """test/H2O.APMO.iter3.py: RHF electrons (6-31G) + two quantum protons
