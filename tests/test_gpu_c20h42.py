@@ -71,7 +71,9 @@ def test_c20h42_mixed_precision_against_oracle(c20h42):
     assert h.stats()["quartets_fp32"] == 0
     h.set_precision("fp64")
     G64 = h.compute_2body_intraspecies_direct(D, -0.5)
-    assert np.max(np.abs(G0 - G64)) < 1e-13
+    # both builds are FP64; they differ only in the order of the global atomic
+    # additions (1e-13 seen on elements of size ~10); FP32 would show 1e-9
+    assert np.max(np.abs(G0 - G64)) < 1e-12
     for tau in (1e-4, 1e-5, 1e-6, 1e-7):
         _case("GPU mixed precision tau=%.0e: C20H42" % tau)
         h.set_precision("mixed", tau)
