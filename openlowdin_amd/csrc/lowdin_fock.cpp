@@ -690,6 +690,7 @@ static TaskSet* coupling_taskset(Species& a, Species& b, hipStream_t st) {
   a.coupling_tasks[b.uid] = {a.generation, b.generation, ts};
   ts->tasks.clear();
   ts->launches.clear();
+  std::vector<double> cost;
   for (int ca = 0; ca < kNumPairClasses; ++ca)
     for (int cb = 0; cb < kNumPairClasses; ++cb) {
       const int na = a.all.off[ca + 1] - a.all.off[ca], nb = b.all.off[cb + 1] - b.all.off[cb];
@@ -706,7 +707,18 @@ static TaskSet* coupling_taskset(Species& a, Species& b, hipStream_t st) {
       sort_by_cost(ts->tasks, from, bras, kets);
       ts->launches.push_back({kPairClassL[cbra][0], kPairClassL[cbra][1], kPairClassL[cket][0],
                               kPairClassL[cket][1], cbra, cket, a_bra ? 1 : 0, from, ts->tasks.size() - from, 0});
+      // estimated cost: primitive quartets x class weight, as for the J/K launches
+      const int lt = kPairClassL[cbra][0] + kPairClassL[cbra][1] + kPairClassL[cket][0] + kPairClassL[cket][1];
+      double c = 0.0;
+      for (size_t t = from; t < ts->tasks.size(); ++t) {
+        const Task& k = ts->tasks[t];
+        c += (double)bras[k.bra].nprim * kets[k.j0].nprim * (k.j1 - k.j0);
+      }
+      cost.push_back(c * (1.0 + lt * lt));
     }
+  ts->order.resize(ts->launches.size());
+  for (size_t k = 0; k < ts->order.size(); ++k) ts->order[k] = k;
+  std::stable_sort(ts->order.begin(), ts->order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
   ts->d_tasks.upload(ts->tasks.data(), ts->tasks.size(), st);
   ts->valid = true;
   return ts;
@@ -1077,13 +1089,17 @@ static void enqueue_coupling(Species& a, Species& b, const double* dDb, double* 
   TaskSet* ts = coupling_taskset(a, b, st);
   if (dRa) launch_fill_zero(dRa, (size_t)a.nbasis * a.nbasis, st);
   if (dRb) launch_fill_zero(dRb, (size_t)b.nbasis * b.nbasis, st);
-  hipStream_t hs = inline_launch ? st : c.side[0];
+  // class launches largest first, round robin over the side streams (the
+  // cooperative kernels have no private segment, the per-lane ones at most
+  // a few hundred bytes per lane); inline: all on the caller's stream
   if (!inline_launch) {
     HIPCHK(hipEventRecord(c.fork, st));
-    HIPCHK(hipStreamWaitEvent(hs, c.fork, 0));
+    for (int q = 0; q < kSideStreams; ++q) HIPCHK(hipStreamWaitEvent(c.side[q], c.fork, 0));
   }
   uint64_t quartets = 0, eris = 0, prims = 0;
-  for (const TaskSet::Launch& L : ts->launches) {
+  for (size_t r = 0; r < ts->launches.size(); ++r) {
+    const TaskSet::Launch& L = ts->launches[ts->order[r]];
+    hipStream_t hs = inline_launch ? st : c.side[r % kSideStreams];
     const bool a_bra = L.bra_is_a != 0;
     Species& sb = a_bra ? a : b;
     Species& sk = a_bra ? b : a;
@@ -1124,10 +1140,7 @@ static void enqueue_coupling(Species& a, Species& b, const double* dDb, double* 
     }
   }
   HIPCHK(hipGetLastError());
-  if (!inline_launch) {
-    HIPCHK(hipEventRecord(c.join[0], hs));
-    HIPCHK(hipStreamWaitEvent(st, c.join[0], 0));
-  }
+  if (!inline_launch) join_side_streams(st);
   if (stt) {
     stt->quartets = quartets;
     stt->eris = eris;
