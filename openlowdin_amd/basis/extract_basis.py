@@ -29,6 +29,14 @@ WANTED = {
     "DZSPNB": ["H_1"],
     "DZSNB": ["H_1"],
     "GRIBAKIN-10S": ["E+"],
+    "NAKAI-7-SP": ["H_1", "H-A_1", "H-B_1"],
+    "SHARON-E-6S2P": ["H"],
+    "SHARON-E+6S2P": ["E+"],
+    "DZSPDN": ["H_1"],
+    "AUG-CC-PVDZ": ["H"],
+    "E+AUG-CC-PVDZ": ["E+"],
+    "STO-3G": ["H", "HE"],
+    "1S1P": ["H_1"],
 }
 
 

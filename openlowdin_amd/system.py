@@ -48,7 +48,9 @@ _BASIS_ALIASES = {
     "CC-PVDZ": "CC-PVDZ", "6-311++G**": "6-311++G.D.P", "6-311++G.D.P": "6-311++G.D.P",
     "6-311G": "6-311G", "6-31G": "6-31G", "NAKAI-5-SPD": "NAKAI-5-SPD",
     "NAKAI-3-S": "NAKAI-3-S", "NAKAI-3-SP": "NAKAI-3-SP", "DZSPNB": "DZSPNB",
-    "GRIBAKIN-10S": "GRIBAKIN-10S",
+    "GRIBAKIN-10S": "GRIBAKIN-10S", "NAKAI-7-SP": "NAKAI-7-SP", "SHARON-E-6S2P": "SHARON-E-6S2P",
+    "SHARON-E+6S2P": "SHARON-E+6S2P", "DZSPDN": "DZSPDN", "AUG-CC-PVDZ": "AUG-CC-PVDZ",
+    "E+AUG-CC-PVDZ": "E+AUG-CC-PVDZ", "STO-3G": "STO-3G", "1S1P": "1S1P",
 }
 
 ATOMIC_NUMBER = {"H": 1, "HE": 2, "LI": 3, "BE": 4, "B": 5, "C": 6, "N": 7, "O": 8, "F": 9, "NE": 10}

@@ -116,6 +116,29 @@ def test_reference_energy_positron_on_gpu(driver, ghost, eref, tol):
     assert abs(E - eref) <= tol
 
 
+@pytest.mark.parametrize("driver", ["host", "device"])
+@pytest.mark.parametrize("name", ["H2O.APMO.UP2", "CO.CISD-dipole", "PsH.FCI", "H-e+H-.DD-CISD", "H3+.FCI",
+                                  "HeH+.NOCI"])
+def test_reference_energy_more_systems_on_gpu(name, driver):
+    """The same six reference energies through the GPU builds, with the
+    host SCF driver and with the device-resident SCF, to 1e-9 Eh."""
+    from openlowdin_amd import scf_device as SD
+    from openlowdin_amd import wavefunction as WF
+    from ref_cases import CASES
+    build, eref, kw = CASES[name]
+    one = kw.get("build_for_one", False)
+    thr = kw.get("overlap_threshold", 1e-8)
+    _case("%s SCF: %s" % (driver, name))
+    if driver == "host":
+        E, _ = WF.run_scf(build(), e_tol=1e-11, d_tol=1e-8, build_for_one_particle=one, overlap_threshold=thr)
+    else:
+        scf = SD.DeviceSCF(build(), build_for_one_particle=one, overlap_threshold=thr)
+        E = scf.run(e_tol=1e-11, d_tol=1e-8)
+        assert scf.converged
+    print("   E = %.12f  ref %.12f  diff %.2e" % (E, eref, E - eref))
+    assert abs(E - eref) <= 1e-9
+
+
 def test_read_coefficients_energy_on_gpu():
     """test/H.APMO.READCOEFF: the SCF started from the plainvec coefficients
     the reference holds (tests/golden/, read as DensityMatrixSCFGuess
