@@ -33,7 +33,8 @@ WANTED = {
     "SHARON-E-6S2P": ["H"],
     "SHARON-E+6S2P": ["E+"],
     "DZSPDN": ["H_1"],
-    "AUG-CC-PVDZ": ["H"],
+    "AUG-CC-PVDZ": ["H", "F"],
+    "PSX-DZ": ["E+"],
     "E+AUG-CC-PVDZ": ["E+"],
     "STO-3G": ["H", "HE"],
     "1S1P": ["H_1"],

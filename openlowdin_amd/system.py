@@ -50,7 +50,7 @@ _BASIS_ALIASES = {
     "NAKAI-3-S": "NAKAI-3-S", "NAKAI-3-SP": "NAKAI-3-SP", "DZSPNB": "DZSPNB",
     "GRIBAKIN-10S": "GRIBAKIN-10S", "NAKAI-7-SP": "NAKAI-7-SP", "SHARON-E-6S2P": "SHARON-E-6S2P",
     "SHARON-E+6S2P": "SHARON-E+6S2P", "DZSPDN": "DZSPDN", "AUG-CC-PVDZ": "AUG-CC-PVDZ",
-    "E+AUG-CC-PVDZ": "E+AUG-CC-PVDZ", "STO-3G": "STO-3G", "1S1P": "1S1P",
+    "E+AUG-CC-PVDZ": "E+AUG-CC-PVDZ", "STO-3G": "STO-3G", "1S1P": "1S1P", "PSX-DZ": "PSX-DZ",
 }
 
 ATOMIC_NUMBER = {"H": 1, "HE": 2, "LI": 3, "BE": 4, "B": 5, "C": 6, "N": 7, "O": 8, "F": 9, "NE": 10}

@@ -118,14 +118,15 @@ def test_reference_energy_positron_on_gpu(driver, ghost, eref, tol):
 
 @pytest.mark.parametrize("driver", ["host", "device"])
 @pytest.mark.parametrize("name", ["H2O.APMO.UP2", "CO.CISD-dipole", "PsH.FCI", "H-e+H-.DD-CISD", "H3+.FCI",
-                                  "HeH+.NOCI"])
+                                  "HeH+.NOCI", "Ps2F2.RHF-customEta"])
 def test_reference_energy_more_systems_on_gpu(name, driver):
-    """The same six reference energies through the GPU builds, with the
-    host SCF driver and with the device-resident SCF, to 1e-9 Eh."""
+    """The same seven reference energies through the GPU builds, with the
+    host SCF driver and with the device-resident SCF, each to its case's
+    tolerance (1e-9 Eh; the checker's 1e-8 for Ps2F2)."""
     from openlowdin_amd import scf_device as SD
     from openlowdin_amd import wavefunction as WF
     from ref_cases import CASES
-    build, eref, kw = CASES[name]
+    build, eref, kw, tol = CASES[name]
     one = kw.get("build_for_one", False)
     thr = kw.get("overlap_threshold", 1e-8)
     _case("%s SCF: %s" % (driver, name))
@@ -136,7 +137,7 @@ def test_reference_energy_more_systems_on_gpu(name, driver):
         E = scf.run(e_tol=1e-11, d_tol=1e-8)
         assert scf.converged
     print("   E = %.12f  ref %.12f  diff %.2e" % (E, eref, E - eref))
-    assert abs(E - eref) <= 1e-9
+    assert abs(E - eref) <= tol
 
 
 def test_read_coefficients_energy_on_gpu():
