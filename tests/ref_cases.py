@@ -61,4 +61,12 @@ CASES = {
         [("F", (0, 0, -1.33)), ("F", (0, 0, 1.33))], "AUG-CC-PVDZ",
         quantum=[("E+", "PSX-DZ", (0, 0, -1.33), 1.0), ("E+", "PSX-DZ", (0, 0, 1.33), 1.0)], extra_electrons=2),
         "E+", 2.0), -199.213740198536, {}, 1e-8),
+    # test/FHF-.NOCI-UHF.lowdin: the FHF- + H_1 (Nakai-3-s) system of the RHF
+    # case with UHF electrons
+    "FHF-.NOCI-UHF": (lambda: S.build_system(
+        [("F", (0, 0, -1.140079)), ("H", (0, 0, 0)), ("F", (0, 0, 1.140079))], "6-31G",
+        quantum=[("H_1", "NAKAI-3-S", (0, 0, 0), H1_MASS)], extra_electrons=1, open_shell=True),
+        -199.396255393556, {}, 1e-9),
+    # test/H2O.BOA.UP2.lowdin: UHF water, 6-311G, point-charge nuclei
+    "H2O.BOA.UP2": (lambda: S.build_system(H2O, "6-311G", open_shell=True), -76.008843007653, {}, 1e-9),
 }

@@ -118,9 +118,9 @@ def test_reference_energy_positron_on_gpu(driver, ghost, eref, tol):
 
 @pytest.mark.parametrize("driver", ["host", "device"])
 @pytest.mark.parametrize("name", ["H2O.APMO.UP2", "CO.CISD-dipole", "PsH.FCI", "H-e+H-.DD-CISD", "H3+.FCI",
-                                  "HeH+.NOCI", "Ps2F2.RHF-customEta"])
+                                  "HeH+.NOCI", "Ps2F2.RHF-customEta", "FHF-.NOCI-UHF", "H2O.BOA.UP2"])
 def test_reference_energy_more_systems_on_gpu(name, driver):
-    """The same seven reference energies through the GPU builds, with the
+    """The same nine reference energies through the GPU builds, with the
     host SCF driver and with the device-resident SCF, each to its case's
     tolerance (1e-9 Eh; the checker's 1e-8 for Ps2F2)."""
     from openlowdin_amd import scf_device as SD

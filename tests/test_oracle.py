@@ -251,9 +251,9 @@ def test_reference_energy_hcn_positron():
 
 
 @pytest.mark.parametrize("name", ["H2O.APMO.UP2", "CO.CISD-dipole", "PsH.FCI", "H-e+H-.DD-CISD", "H3+.FCI",
-                                  "HeH+.NOCI", "Ps2F2.RHF-customEta"])
+                                  "HeH+.NOCI", "Ps2F2.RHF-customEta", "FHF-.NOCI-UHF", "H2O.BOA.UP2"])
 def test_reference_energy_more_systems(name):
-    """HF energies of seven more of the reference's tests (UHF, quantum
+    """HF energies of nine more of the reference's tests (UHF, quantum
     protons in one or two species, positrons, one-particle builds, overlap
     threshold, a custom eta): the oracle SCF converged to 1e-11 lands within
     each case's tolerance (the reference's runs stop at their own energy
