@@ -429,6 +429,10 @@ int lowdin_scf_batch_remove(void* eng, int job);
  * stream (the default forks the class launches over the library's side
  * streams); engines on different streams then overlap freely. */
 int lowdin_scf_batch_options(void* eng, int use_graph, int inline_launch);
+/* Overlap eigenvalues kept by the orthogonaliser of the jobs added from now
+ * on (OVERLAP_EIGEN_THRESHOLD, default 1e-8; lowdin_scf_set_overlap_threshold
+ * for one SCF object).  threshold > 0. */
+int lowdin_scf_batch_set_overlap_threshold(void* eng, double threshold);
 /* One SCF iteration of every job; waits for the per-job scalars.  Returns 0
  * on success (a job whose diagonalisation failed is marked, the others go on). */
 int lowdin_scf_batch_step(void* eng);

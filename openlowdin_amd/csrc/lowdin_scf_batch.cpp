@@ -378,6 +378,7 @@ struct Engine {
   uint64_t version = 1, batch_version = 0, graph_version = 0, warm_version = 0;
   bool use_graph = true;
   bool inline_launch = false;   // class launches on this engine's stream, not the library's side streams
+  double overlap_threshold = 1e-8;   // OVERLAP_EIGEN_THRESHOLD for the jobs added from now on
   bool in_flight = false;       // an iteration enqueued and not yet collected
   hipGraphExec_t g1 = nullptr, g2 = nullptr;
   std::vector<void*> b_lints;
@@ -713,7 +714,7 @@ void setup_species(Engine& e, const double* H, const double* S, int n, std::vect
   BCHK(hipStreamSynchronize(st));
   if (info != 0) throw std::runtime_error("overlap diagonalisation failed");
   int col0 = 0;
-  while (col0 < n && !(w[col0] > 1e-8)) ++col0;
+  while (col0 < n && !(w[col0] >= e.overlap_threshold)) ++col0;
   m = n - col0;
   hX.assign(nn, 0.0);   // X = U_kept s^-1/2, n x m in an n x n slot (column-major)
   for (int c = 0; c < m; ++c) {
@@ -909,6 +910,16 @@ int lowdin_scf_batch_remove(void* eng, int job) {
     lowdin::set_last_error(std::string("scf_batch_remove: ") + ex.what());
     return -1;
   }
+}
+
+int lowdin_scf_batch_set_overlap_threshold(void* eng, double threshold) {
+  if (!(threshold > 0.0)) {
+    lowdin::set_last_error("scf_batch_set_overlap_threshold: threshold must be positive");
+    return -1;
+  }
+  as_engine(eng)->overlap_threshold = threshold;
+  lowdin::clear_last_error();
+  return 0;
 }
 
 int lowdin_scf_batch_options(void* eng, int use_graph, int inline_launch) {

@@ -33,6 +33,8 @@ def _bind():
     L.lowdin_scf_batch_remove.restype = C.c_int
     L.lowdin_scf_batch_remove.argtypes = [vp, C.c_int]
     L.lowdin_scf_batch_options.argtypes = [vp, C.c_int, C.c_int]
+    L.lowdin_scf_batch_set_overlap_threshold.restype = C.c_int
+    L.lowdin_scf_batch_set_overlap_threshold.argtypes = [vp, C.c_double]
     L.lowdin_scf_batch_step.restype = C.c_int
     L.lowdin_scf_batch_step.argtypes = [vp]
     for name in ("lowdin_scf_batch_step_begin", "lowdin_scf_batch_step_query", "lowdin_scf_batch_step_end"):
@@ -60,9 +62,12 @@ class BatchSCF:
         self._keep: Dict[int, tuple] = {}    # job -> (handles, system) kept alive while the job is in
 
     def add(self, system, objs: Optional[dict] = None, mixed_tau: Optional[float] = None,
-            build_for_one_particle: bool = False, hcore=None, overlap=None) -> int:
+            build_for_one_particle: bool = False, hcore=None, overlap=None,
+            overlap_threshold: float = 1e-8) -> int:
         """Admit a job; returns its id.  Raises LowdinFockError when the
-        library refuses it (the engine is unchanged then)."""
+        library refuses it (the engine is unchanged then).  overlap_threshold:
+        overlap eigenvalues the job's orthogonalisers keep
+        (OVERLAP_EIGEN_THRESHOLD)."""
         objs = {} if objs is None else objs
         ns = len(system.species)
         handles = [LI._instance(objs, system, k + 1) for k in range(ns)]
@@ -85,6 +90,9 @@ class BatchSCF:
         Hp = (C.POINTER(C.c_double) * ns)(*[LI._dptr(a) for a in Hs])
         Sp = (C.POINTER(C.c_double) * ns)(*[LI._dptr(a) for a in Ss])
         arr = (C.c_void_p * ns)(*[h.this for h in handles])
+        if LI.lib().lowdin_scf_batch_set_overlap_threshold(self.this, float(overlap_threshold)) != 0:
+            LI._check("scf_batch_set_overlap_threshold")
+            raise LI.LowdinFockError("scf_batch_set_overlap_threshold failed")
         j = LI.lib().lowdin_scf_batch_add(self.this, ns, arr, desc, float(system.point_charge_repulsion()), Hp, Sp)
         if j < 0:
             LI._check("scf_batch_add")
@@ -147,11 +155,14 @@ class BatchSCF:
 
 
 def run_batch(systems, e_tol: float = 1e-9, d_tol: float = 1e-7, max_iter: int = 200, use_graph: bool = True,
-              mixed_tau: Optional[float] = None):
+              mixed_tau: Optional[float] = None, options=None):
     """Converge every system together; returns [(energy, iterations, converged)]
-    in input order.  A job leaves the batch when converged or failed."""
+    in input order.  A job leaves the batch when converged or failed.
+    options: per system, keyword arguments of BatchSCF.add (e.g.
+    build_for_one_particle, overlap_threshold), or None."""
     eng = BatchSCF(use_graph=use_graph)
-    jobs = [eng.add(s, mixed_tau=mixed_tau) for s in systems]
+    opts = options or [None] * len(systems)
+    jobs = [eng.add(s, mixed_tau=mixed_tau, **(o or {})) for s, o in zip(systems, opts)]
     res = [None] * len(systems)
     prev = {}
     live = dict(enumerate(jobs))

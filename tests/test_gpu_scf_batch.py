@@ -50,6 +50,35 @@ def test_reference_energies_in_one_batch():
         assert abs(E - eref) <= 1e-9, name
 
 
+def test_reference_energies_mixed_options_in_one_batch():
+    """Reference systems with their own orthogonaliser thresholds and
+    one-particle builds in one batch: HCN.e+ and H3+.FCI (overlap
+    threshold 1e-6), PsH.FCI (two-particle matrices for one particle),
+    CO.CISD-dipole (UHF) -- each to its reference energy."""
+    from openlowdin_amd import scf_batch as SB
+    from ref_cases import CASES
+    hcn = [("H", (0, 0, 0)), ("C", (0, 0, 1.059)), ("N", (0, 0, 2.186))]
+    jobs = [("HCN.e+", lambda: S.build_system(hcn, "6-311++G.D.P",
+                                              quantum=[("E+", "GRIBAKIN-10S", x, 1.0) for _, x in hcn],
+                                              add_particles={"E+": -2}),
+             -92.901811855736, {"overlap_threshold": 1e-6}, 1e-9)]
+    for name in ("H3+.FCI", "PsH.FCI", "CO.CISD-dipole"):
+        build, eref, kw, tol = CASES[name]
+        opts = {}
+        if kw.get("build_for_one"):
+            opts["build_for_one_particle"] = True
+        if "overlap_threshold" in kw:
+            opts["overlap_threshold"] = kw["overlap_threshold"]
+        jobs.append((name, build, eref, opts, tol))
+    _case("batched SCF: %d reference systems with per-job options" % len(jobs))
+    res = SB.run_batch([b() for _, b, _, _, _ in jobs], e_tol=1e-11, d_tol=1e-8, max_iter=300,
+                       options=[o for _, _, _, o, _ in jobs])
+    for (name, _, eref, _, tol), (E, it, conv) in zip(jobs, res):
+        print("   %-16s E = %.12f  diff %.2e  (%d iterations)" % (name, E, E - eref, it), flush=True)
+        assert conv, name
+        assert abs(E - eref) <= tol, name
+
+
 def _energies(use_graph, systems, steps):
     from openlowdin_amd import scf_batch as SB
     eng = SB.BatchSCF(use_graph=use_graph)
