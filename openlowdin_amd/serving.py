@@ -71,6 +71,7 @@ class Job:
     handle: Optional[int] = None       # job id inside the lane's engine
     e_old: Optional[float] = None
     objs: dict = field(default_factory=dict)
+    options: dict = field(default_factory=dict)   # BatchSCF.add keywords (overlap_threshold, build_for_one_particle)
 
 
 # lanes by total basis functions of a job: small, medium, large
@@ -142,7 +143,7 @@ class ScfEngine:
                 continue
             if len(self.active) < self.max_jobs and self.used + job.bytes <= self.budget:
                 try:
-                    job.handle = lane.eng.add(job.system, objs=job.objs, mixed_tau=self.mixed_tau)
+                    job.handle = lane.eng.add(job.system, objs=job.objs, mixed_tau=self.mixed_tau, **job.options)
                 except LI.LowdinFockError as exc:
                     job.admitted = now
                     self._finish(job, "rejected", str(exc))
