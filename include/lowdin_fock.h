@@ -412,7 +412,7 @@ void* lowdin_scf_stream(void* scf);
  * assembly, traces, error maxima and the DIIS history, B matrix, solve and
  * extrapolation (the DIIS state stays on the device).  While the jobs are
  * unchanged an iteration replays two captured HIP graphs around the
- * diagonalisation.  Same iteration as lowdin_scf_* (no ghost species).  No
+ * diagonalisation.  Same iteration as lowdin_scf_* (ghost species included).  No
  * reference counterpart: openLOWDIN iterates one molecule per process. */
 void* lowdin_scf_batch_new(void* stream);
 void lowdin_scf_batch_del(void* eng);

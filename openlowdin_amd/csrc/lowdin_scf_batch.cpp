@@ -309,6 +309,12 @@ __global__ void b_fill_kernel(double* p, size_t count, double v) {
     p[e] = v;
 }
 
+// F -= Cg over every slot (the ghost species' coupling leaves the Fock matrix).
+__global__ void b_sub_kernel(double* F, const double* Cg, size_t count) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < count; e += (size_t)gridDim.x * blockDim.x)
+    F[e] -= Cg[e];
+}
+
 __global__ void b_job_reset_kernel(JobDev* jobs, int j) {
   JobDev& J = jobs[j];
   J.nhist = 0;
@@ -331,13 +337,14 @@ struct SlotInfo {
   double factor = -0.5, charge = -1.0, eta = 2.0;
   int nocc = 0;
   bool build_two = true;
+  bool ghost = false;   // SCF_GHOST_SPECIES: its coupling stays out of the job's other Fock matrices
 };
 
 struct Bucket {
   int n = 0, m = 0;
   int cap = 0;
   std::vector<SlotInfo> slots;
-  DArr<double> H, S, X, D, G, Cs, F, T, R, C, Fp, E, W, occ, qq, Fh, Eh, sc;
+  DArr<double> H, S, X, D, G, Cs, Cg, F, T, R, C, Fp, E, W, occ, qq, Fh, Eh, sc;   // Cg: coupling from ghost species
   DArr<int> info, slot_job;
   size_t nn() const { return (size_t)n * n; }
   size_t mm() const { return (size_t)m * m; }
@@ -426,7 +433,7 @@ int take_slot(Engine& e, Bucket& b) {
   const int cap = std::max(4, 2 * old);
   const size_t nn = b.nn(), mm = b.mm();
   hipStream_t st = e.stream;
-  for (DArr<double>* a : {&b.H, &b.S, &b.X, &b.D, &b.G, &b.Cs, &b.F, &b.T, &b.R, &b.C})
+  for (DArr<double>* a : {&b.H, &b.S, &b.X, &b.D, &b.G, &b.Cs, &b.Cg, &b.F, &b.T, &b.R, &b.C})
     a->grow((size_t)cap * nn, (size_t)old * nn, st);
   b.Fp.grow((size_t)cap * mm, (size_t)old * mm, st);
   b.E.grow((size_t)cap * mm, (size_t)old * mm, st);
@@ -438,13 +445,13 @@ int take_slot(Engine& e, Bucket& b) {
   b.Eh.grow((size_t)cap * K * mm, (size_t)old * K * mm, st);
   b.info.grow(cap, old, st);
   b.slot_job.grow(cap, old, st);
-  e.bytes += (size_t)(cap - old) * ((10 + K) * nn + (2 + K) * mm + 2 * b.m + kSlotScal + 1) * sizeof(double);
+  e.bytes += (size_t)(cap - old) * ((11 + K) * nn + (2 + K) * mm + 2 * b.m + kSlotScal + 1) * sizeof(double);
   b.cap = cap;
   b.slots.resize(cap);
   // new slots: nobody's, zero G (species without a two-particle build keep it)
   std::vector<int> owners(cap - old, -1);
   BCHK(hipMemcpyAsync(b.slot_job.p + old, owners.data(), owners.size() * sizeof(int), hipMemcpyHostToDevice, st));
-  for (DArr<double>* a : {&b.G, &b.Cs, &b.D, &b.H, &b.S, &b.X, &b.F, &b.Fp, &b.E})
+  for (DArr<double>* a : {&b.G, &b.Cs, &b.Cg, &b.D, &b.H, &b.S, &b.X, &b.F, &b.Fp, &b.E})
     hipLaunchKernelGGL(b_fill_kernel, dim3(grid1((size_t)(cap - old) * (a == &b.Fp || a == &b.E ? mm : nn))),
                        dim3(256), 0, st, a->p + (size_t)old * (a == &b.Fp || a == &b.E ? mm : nn),
                        (size_t)(cap - old) * (a == &b.Fp || a == &b.E ? mm : nn), 0.0);
@@ -498,10 +505,17 @@ void enqueue_part1(Engine& e) {
   if (e.fock_batch)
     lowdin::internal_enqueue_batch_build(e.fock_batch, e.b_D.data(), e.b_G.data(), e.b_fac.data(), st,
                                          e.inline_launch);
+  // a job with a ghost species: its partners' Fock matrices leave out that
+  // coupling (Cg), the energy keeps it (WaveFunction.f90:1502-1503, 1980-1986)
+  bool any_ghost = false;
+  for (const auto& bp : e.buckets)
+    for (const SlotInfo& si : bp->slots) any_ghost = any_ghost || (si.job >= 0 && si.ghost);
   for (auto& bp : e.buckets) {
     Bucket& b = *bp;
     const int u = b.used();
     if (u) hipLaunchKernelGGL(b_fill_kernel, dim3(grid1((size_t)u * b.nn())), dim3(256), 0, st, b.Cs.p, (size_t)u * b.nn(), 0.0);
+    if (u && any_ghost)
+      hipLaunchKernelGGL(b_fill_kernel, dim3(grid1((size_t)u * b.nn())), dim3(256), 0, st, b.Cg.p, (size_t)u * b.nn(), 0.0);
   }
   // couplings of the jobs with several species (one fused pass per pair)
   for (const JobHost& J : e.jobs) {
@@ -520,6 +534,8 @@ void enqueue_part1(Engine& e) {
         const double qq = ia.charge * ic.charge;
         lowdin::launch_sym_add(ba.Cs.p + (size_t)sa * ba.nn(), Ra, ba.n, qq, st);
         lowdin::launch_sym_add(bc.Cs.p + (size_t)sc * bc.nn(), Rc, bc.n, qq, st);
+        if (ic.ghost) lowdin::launch_sym_add(ba.Cg.p + (size_t)sa * ba.nn(), Ra, ba.n, qq, st);
+        if (ia.ghost) lowdin::launch_sym_add(bc.Cg.p + (size_t)sc * bc.nn(), Rc, bc.n, qq, st);
       }
   }
   hipLaunchKernelGGL(b_diis_begin_kernel, dim3((e.job_cap + 63) / 64), dim3(64), 0, st, e.d_jobs.p, e.d_active.p,
@@ -533,6 +549,8 @@ void enqueue_part1(Engine& e) {
     const size_t nn = b.nn(), mm = b.mm();
     hipLaunchKernelGGL(b_assemble_kernel, dim3(grid1(nn), u), dim3(256), 0, st, b.F.p, b.H.p, b.G.p, b.Cs.p, b.qq.p,
                        nn, u);
+    if (any_ghost)
+      hipLaunchKernelGGL(b_sub_kernel, dim3(grid1((size_t)u * nn)), dim3(256), 0, st, b.F.p, b.Cg.p, (size_t)u * nn);
     hipLaunchKernelGGL(b_traces_kernel, dim3(u, 3), dim3(256), 0, st, b.D.p, b.H.p, b.G.p, b.Cs.p, nn, b.sc.p);
     // error X^T (F D S - S D F) X
     gemm_sb(e, N, N, n, n, n, b.F.p, n, nn, b.D.p, n, nn, 0.0, b.T.p, n, nn, u);
@@ -777,7 +795,6 @@ int add_job(Engine& e, int nspecies, void* const* lints, const lowdin_scf_specie
   try {
     for (int s = 0; s < nspecies; ++s) {
       const int n = LibintInterface_get_nbasis(lints[s]);
-      if (desc[s].ghost) throw std::runtime_error("ghost species are not supported by the batched SCF");
       std::vector<double> hX;
       int m = 0;
       setup_species(e, H[s], S[s], n, hX, m);
@@ -796,6 +813,7 @@ int add_job(Engine& e, int nspecies, void* const* lints, const lowdin_scf_specie
       si.eta = desc[s].eta;
       si.nocc = desc[s].nocc;
       si.build_two = desc[s].build_two != 0;
+      si.ghost = desc[s].ghost != 0;
       placed.push_back({bi, k});
       const size_t nn = b.nn();
       BCHK(hipMemcpyAsync(b.H.p + k * nn, H[s], nn * sizeof(double), hipMemcpyHostToDevice, st));

@@ -63,11 +63,13 @@ class BatchSCF:
 
     def add(self, system, objs: Optional[dict] = None, mixed_tau: Optional[float] = None,
             build_for_one_particle: bool = False, hcore=None, overlap=None,
-            overlap_threshold: float = 1e-8) -> int:
+            overlap_threshold: float = 1e-8, ghost: str = "") -> int:
         """Admit a job; returns its id.  Raises LowdinFockError when the
         library refuses it (the engine is unchanged then).  overlap_threshold:
         overlap eigenvalues the job's orthogonalisers keep
-        (OVERLAP_EIGEN_THRESHOLD)."""
+        (OVERLAP_EIGEN_THRESHOLD); ghost: name of the job's SCF ghost species
+        (its coupling stays out of the other species' Fock matrices, in the
+        energy)."""
         objs = {} if objs is None else objs
         ns = len(system.species)
         handles = [LI._instance(objs, system, k + 1) for k in range(ns)]
@@ -84,7 +86,7 @@ class BatchSCF:
             desc[k].charge = sp.charge
             desc[k].factor = sp.exchange_factor
             desc[k].build_two = 1 if (sp.n_particles > 1 or build_for_one_particle) else 0
-            desc[k].ghost = 0
+            desc[k].ghost = 1 if (ghost and sp.name == ghost) else 0
         Hs = [np.asfortranarray(h, dtype=np.float64) for h in hcore]
         Ss = [np.asfortranarray(s, dtype=np.float64) for s in overlap]
         Hp = (C.POINTER(C.c_double) * ns)(*[LI._dptr(a) for a in Hs])

@@ -51,9 +51,10 @@ def test_reference_energies_in_one_batch():
 
 
 def test_reference_energies_mixed_options_in_one_batch():
-    """Reference systems with their own orthogonaliser thresholds and
-    one-particle builds in one batch: HCN.e+ and H3+.FCI (overlap
-    threshold 1e-6), PsH.FCI (two-particle matrices for one particle),
+    """Reference systems with their own orthogonaliser thresholds, ghost
+    species and one-particle builds in one batch: HCN.e+ and H3+.FCI
+    (overlap threshold 1e-6), HCN.e+.FT (the positron as SCF ghost species,
+    the checker's 1e-8), PsH.FCI (two-particle matrices for one particle),
     CO.CISD-dipole (UHF) -- each to its reference energy."""
     from openlowdin_amd import scf_batch as SB
     from ref_cases import CASES
@@ -62,6 +63,7 @@ def test_reference_energies_mixed_options_in_one_batch():
                                               quantum=[("E+", "GRIBAKIN-10S", x, 1.0) for _, x in hcn],
                                               add_particles={"E+": -2}),
              -92.901811855736, {"overlap_threshold": 1e-6}, 1e-9)]
+    jobs.append(("HCN.e+.FT", jobs[0][1], -92.901807583131, {"overlap_threshold": 1e-6, "ghost": "E+"}, 1e-8))
     for name in ("H3+.FCI", "PsH.FCI", "CO.CISD-dipole"):
         build, eref, kw, tol = CASES[name]
         opts = {}
