@@ -433,6 +433,11 @@ int lowdin_scf_batch_options(void* eng, int use_graph, int inline_launch);
  * on (OVERLAP_EIGEN_THRESHOLD, default 1e-8; lowdin_scf_set_overlap_threshold
  * for one SCF object).  threshold > 0. */
 int lowdin_scf_batch_set_overlap_threshold(void* eng, double threshold);
+/* lowdin_fock_batch_autotune on the engine's own batched build (every
+ * species with a two-particle build of every job, at the jobs' current
+ * densities); the choices stay with the engine and apply to the build after
+ * jobs join or leave.  Call between iterations; waits. */
+int lowdin_scf_batch_autotune(void* eng, int reps);
 /* One SCF iteration of every job; waits for the per-job scalars.  Returns 0
  * on success (a job whose diagonalisation failed is marked, the others go on). */
 int lowdin_scf_batch_step(void* eng);

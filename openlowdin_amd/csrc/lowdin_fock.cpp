@@ -2155,7 +2155,8 @@ int lowdin_fock_batch_launch_profile(void* batch, int cap, int* classes, float* 
 // Per class launch: try each candidate workgroup target on serialised,
 // event-timed builds and keep the fastest (ties to the smaller target).
 static int plan_autotune(lowdin::BuildPlan& p, const double* d_dens, double* d_result, const double* factors,
-                         hipStream_t stream, int reps) {
+                         hipStream_t stream, int reps, const double* const* dD_each = nullptr,
+                         double* const* dG_each = nullptr) {
   lowdin::Context& c = lowdin::ctx();
   std::lock_guard<std::mutex> lk(c.mu);
   lowdin::ensure_context();
@@ -2185,7 +2186,7 @@ static int plan_autotune(lowdin::BuildPlan& p, const double* d_dens, double* d_r
       p.valid = false;
       std::vector<double> ms;
       for (int r = 0; r < reps + 1; ++r) {
-        lowdin::enqueue_plan(p, d_dens, d_result, factors, st);
+        lowdin::enqueue_plan(p, d_dens, d_result, factors, st, dD_each, dG_each);
         lowdin::collect_plan_stats(p, st, lowdin::now_ms());
         if (r == 0) {
           ms.assign(p.ts.launches.size(), 0.0);
@@ -2294,6 +2295,33 @@ void internal_enqueue_batch_build(void* batch, const double* const* dD, double* 
   p.inline_launch = inline_launch;
   if (p.use_graph || p.time_launches) throw std::runtime_error("batched SCF builds run without graph or timing");
   enqueue_plan(p, nullptr, nullptr, factors, st, dD, dG, false);
+}
+
+// The autotuner on a batch whose densities and results are per-molecule
+// device buffers (the batched SCF's slots); the results' contents are
+// overwritten.  Waits.
+void internal_batch_autotune(void* batch, const double* const* dD, double* const* dG, const double* factors,
+                             hipStream_t st, int reps) {
+  BuildPlan& p = static_cast<lowdin_fock_batch_t*>(batch)->plan;
+  plan_autotune(p, nullptr, nullptr, factors, st, std::max(reps, 1), dD, dG);
+}
+
+// Per-class kernel choices of a batch's plan (kNumClassPairs each): read, and
+// set on another batch of the same kind of molecules.
+void internal_batch_get_tuning(void* batch, int* kind, int* target) {
+  const BuildPlan& p = static_cast<lowdin_fock_batch_t*>(batch)->plan;
+  for (int k = 0; k < kNumClassPairs; ++k) {
+    kind[k] = p.kind[k];
+    target[k] = p.target_wg[k];
+  }
+}
+void internal_batch_set_tuning(void* batch, const int* kind, const int* target) {
+  BuildPlan& p = static_cast<lowdin_fock_batch_t*>(batch)->plan;
+  for (int k = 0; k < kNumClassPairs; ++k) {
+    p.kind[k] = kind[k];
+    p.target_wg[k] = target[k];
+  }
+  p.valid = false;
 }
 
 void internal_enqueue_coupling_pair(void* a, void* b, const double* dDa, const double* dDb, double* dRa, double* dRb,

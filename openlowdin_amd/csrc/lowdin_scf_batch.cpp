@@ -49,6 +49,10 @@ void internal_enqueue_batch_build(void* batch, const double* const* dD, double* 
                                   hipStream_t st, bool inline_launch);
 void internal_enqueue_coupling_pair(void* a, void* b, const double* dDa, const double* dDb, double* dRa, double* dRb,
                                     hipStream_t st, bool inline_launch);
+void internal_batch_autotune(void* batch, const double* const* dD, double* const* dG, const double* factors,
+                             hipStream_t st, int reps);
+void internal_batch_get_tuning(void* batch, int* kind, int* target);
+void internal_batch_set_tuning(void* batch, const int* kind, const int* target);
 }  // namespace lowdin
 
 namespace {
@@ -379,6 +383,10 @@ struct Engine {
   size_t h_cap = 0;
   hipEvent_t done = nullptr;
   void* fock_batch = nullptr;
+  // per-class kernel choices of the autotuner, applied to every later
+  // version of the batched build (molecules leave as they converge)
+  bool tuned = false;
+  int tune_kind[21] = {}, tune_target[21] = {};
   // version counts changes of the job set; graphs are captured only for a
   // version whose eager iteration has run (that one prepares the build's
   // task lists and buffers, which allocate and must not be captured)
@@ -495,6 +503,7 @@ void ensure_fock_batch(Engine& e) {
   if (!e.b_lints.empty()) {
     e.fock_batch = lowdin_fock_batch_new((int)e.b_lints.size(), e.b_lints.data());
     if (!e.fock_batch) throw std::runtime_error(std::string("batch build: ") + lowdin_fock_last_error());
+    if (e.tuned) lowdin::internal_batch_set_tuning(e.fock_batch, e.tune_kind, e.tune_target);
   }
   e.batch_version = e.version;
 }
@@ -926,6 +935,28 @@ int lowdin_scf_batch_remove(void* eng, int job) {
     return 0;
   } catch (const std::exception& ex) {
     lowdin::set_last_error(std::string("scf_batch_remove: ") + ex.what());
+    return -1;
+  }
+}
+
+int lowdin_scf_batch_autotune(void* eng, int reps) {
+  try {
+    Engine& e = *as_engine(eng);
+    if (e.in_flight) throw std::runtime_error("an iteration is in flight");
+    ensure_fock_batch(e);
+    if (e.fock_batch) {
+      lowdin::internal_batch_autotune(e.fock_batch, e.b_D.data(), e.b_G.data(), e.b_fac.data(), e.stream, reps);
+      lowdin::internal_batch_get_tuning(e.fock_batch, e.tune_kind, e.tune_target);
+      e.tuned = true;
+      // the slots' G buffers were overwritten (the next iteration rebuilds
+      // them first); graphs hold the old task lists
+      drop_graphs(e);
+      e.warm_version = 0;
+    }
+    lowdin::clear_last_error();
+    return 0;
+  } catch (const std::exception& ex) {
+    lowdin::set_last_error(std::string("scf_batch_autotune: ") + ex.what());
     return -1;
   }
 }

@@ -35,6 +35,8 @@ def _bind():
     L.lowdin_scf_batch_options.argtypes = [vp, C.c_int, C.c_int]
     L.lowdin_scf_batch_set_overlap_threshold.restype = C.c_int
     L.lowdin_scf_batch_set_overlap_threshold.argtypes = [vp, C.c_double]
+    L.lowdin_scf_batch_autotune.restype = C.c_int
+    L.lowdin_scf_batch_autotune.argtypes = [vp, C.c_int]
     L.lowdin_scf_batch_step.restype = C.c_int
     L.lowdin_scf_batch_step.argtypes = [vp]
     for name in ("lowdin_scf_batch_step_begin", "lowdin_scf_batch_step_query", "lowdin_scf_batch_step_end"):
@@ -102,6 +104,13 @@ class BatchSCF:
         self._keep[j] = (handles, objs, system)
         return j
 
+    def autotune(self, reps: int = 1):
+        """Per-class kernel shapes of the engine's batched build chosen by
+        timing (kept for the builds after jobs join or leave)."""
+        if LI.lib().lowdin_scf_batch_autotune(self.this, reps) != 0:
+            LI._check("scf_batch_autotune")
+            raise LI.LowdinFockError("scf_batch_autotune failed")
+
     def remove(self, job: int):
         if LI.lib().lowdin_scf_batch_remove(self.this, job) != 0:
             LI._check("scf_batch_remove")
@@ -157,14 +166,17 @@ class BatchSCF:
 
 
 def run_batch(systems, e_tol: float = 1e-9, d_tol: float = 1e-7, max_iter: int = 200, use_graph: bool = True,
-              mixed_tau: Optional[float] = None, options=None):
+              mixed_tau: Optional[float] = None, options=None, autotune: bool = False):
     """Converge every system together; returns [(energy, iterations, converged)]
     in input order.  A job leaves the batch when converged or failed.
     options: per system, keyword arguments of BatchSCF.add (e.g.
-    build_for_one_particle, overlap_threshold), or None."""
+    build_for_one_particle, overlap_threshold), or None.  autotune: tune
+    the batched build's kernel shapes once the jobs are in."""
     eng = BatchSCF(use_graph=use_graph)
     opts = options or [None] * len(systems)
     jobs = [eng.add(s, mixed_tau=mixed_tau, **(o or {})) for s, o in zip(systems, opts)]
+    if autotune:
+        eng.autotune()
     res = [None] * len(systems)
     prev = {}
     live = dict(enumerate(jobs))

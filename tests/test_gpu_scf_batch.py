@@ -40,10 +40,13 @@ def _case(msg):
     print("\n[case] " + msg, flush=True)
 
 
-def test_reference_energies_in_one_batch():
+@pytest.mark.parametrize("autotune", [False, True], ids=["default", "autotuned"])
+def test_reference_energies_in_one_batch(autotune):
+    """Also with the engine's batched build autotuned: the kernel shapes
+    change, the energies do not."""
     from openlowdin_amd import scf_batch as SB
-    _case("batched SCF: 4 reference systems in one batch")
-    res = SB.run_batch([b() for _, b, _ in REFERENCE], e_tol=1e-11, d_tol=1e-9, max_iter=300)
+    _case("batched SCF: 4 reference systems in one batch (autotune %s)" % autotune)
+    res = SB.run_batch([b() for _, b, _ in REFERENCE], e_tol=1e-11, d_tol=1e-9, max_iter=300, autotune=autotune)
     for (name, _, eref), (E, it, conv) in zip(REFERENCE, res):
         print("   %-22s E = %.12f  diff %.2e  (%d iterations)" % (name, E, E - eref, it), flush=True)
         assert conv, name
