@@ -130,8 +130,34 @@ class ScfEngine:
         self.waiting.append(job)
 
     def _lane_of(self, job: Job) -> int:
-        n = sum(sp.nbasis for sp in job.system.species)
+        return self._lane_of_system(job.system)
+
+    def _lane_of_system(self, system) -> int:
+        n = sum(sp.nbasis for sp in system.species)
         return next(k for k, lim in enumerate(self.limits) if n <= lim)
+
+    def tune(self, systems, reps: int = 1) -> float:
+        """Before serving: the kernel shapes of every lane's batched build
+        chosen by the autotuner on the given representative systems of the
+        lane's size range (added, tuned, removed; the shapes stay with the
+        lane's engine).  Restarts the engine clock.  Returns the seconds
+        spent."""
+        if self.waiting or self.active:
+            raise RuntimeError("tune before submitting requests")
+        t0 = time.perf_counter()
+        for lane in self.lanes:
+            mine = [s for s in systems if self._lane_of_system(s) == lane.k]
+            if not mine:
+                continue
+            ids = [lane.eng.add(s, mixed_tau=self.mixed_tau) for s in mine]
+            try:
+                lane.eng.autotune(reps)
+            finally:
+                for j in ids:
+                    lane.eng.remove(j)
+        spent = time.perf_counter() - t0
+        self.t0 = time.perf_counter()
+        return spent
 
     def _admit(self, lane: _Lane):
         """First come first served into an idle lane while memory and slots allow."""

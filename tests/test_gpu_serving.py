@@ -59,6 +59,26 @@ def test_alone_equals_mix():
     assert sum(1 for l in eng.lanes if l.iterations) >= 2           # more than one lane ran
 
 
+def test_tuned_lanes_serve_the_same_energies():
+    """Lanes tuned on representative jobs before serving: the kernel shapes
+    change, every job's energy does not."""
+    from openlowdin_amd import serving as SV
+    jobs = _jobs()
+    _case("serving: %d jobs with tuned lanes against untuned" % len(jobs))
+    ref, _, _ = _run(jobs, [0.0] * len(jobs))
+    eng = SV.ScfEngine()
+    spent = eng.tune([s for _, s in jobs])
+    for k, (label, sysm) in enumerate(jobs):
+        eng.submit(SV.Job(k, label, sysm, 0.0))
+    eng.run(timeout=300)
+    res = {j.jid: j for j in eng.done}
+    eng.close()
+    print("   tuning %.2f s" % spent)
+    for k, (label, _) in enumerate(jobs):
+        assert res[k].converged, label
+        assert abs(res[k].energy - ref[k].energy) < 1e-8, label
+
+
 def test_rejected_and_refused_jobs_leave_others_unchanged():
     from openlowdin_amd import serving as SV
     jobs = _jobs()[:3]
